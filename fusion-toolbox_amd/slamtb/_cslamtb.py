@@ -1,0 +1,337 @@
+"""Drop-in replacement of the reference's native module ``slamtb._cslamtb``
+(src/_cslamtb.cpp:22-53) for the ICP hot path, backed by libslamtb_hip.so.
+
+Same class names, static-method names, positional arguments and in-place
+output convention as the pybind11 bindings:
+
+* ``ICPJacobian`` -- src/icp_jacobian.cpp:7-13
+* ``CorrespondenceMap`` -- src/matching.cpp:12-16
+* ``Processing``, ``EstimateNormalsMethod``, ``DownsampleXYZMethod`` --
+  src/processing.cpp:7-22
+* ``ExpRtToMatrixOp``, ``MatrixToExpRtOp`` -- src/se3.cpp:6-15
+
+Differences, all stricter or documented: tensors must live on a HIP device
+(the reference also had a CPU instantiation; this build has no CPU path),
+output tensors must be contiguous, and ``kcam``/``rt_cam`` are cast to the
+source dtype instead of failing the accessor type check.
+"""
+import enum
+
+import torch
+
+from . import _lib
+from ._lib import check, dtype_code, lib, ptr, require_contiguous_out, require_gpu, stream
+
+
+class EstimateNormalsMethod(enum.IntEnum):
+    CentralDifferences = 0
+    Average8 = 1
+
+
+class DownsampleXYZMethod(enum.IntEnum):
+    Nearest = 0
+
+
+def _in(t, dtype=None):
+    if dtype is not None and t.dtype != dtype:
+        t = t.to(dtype)
+    return t.contiguous()
+
+
+def _mask(t):
+    if t.dtype != torch.bool and t.dtype != torch.uint8:
+        t = t.to(torch.bool)
+    return t.contiguous()
+
+
+def _rt(rt_cam, dtype):
+    rt = _in(rt_cam, dtype)
+    if rt.dim() != 2 or rt.size(1) != 4 or rt.size(0) not in (3, 4):
+        raise RuntimeError("rt_cam must be a [3,4] or [4,4] matrix")
+    return rt
+
+
+def _kcam(kcam, dtype):
+    k = _in(kcam, dtype)
+    if k.dim() != 2 or k.size(0) < 2 or k.size(1) != 3:
+        raise RuntimeError("kcam must be a [3,3] (or [2,3]) matrix")
+    if k.size(0) == 2:
+        k = torch.cat([k, torch.tensor([[0, 0, 1]], dtype=k.dtype, device=k.device)])
+    return k
+
+
+class CorrespondenceMap:
+    @staticmethod
+    def compute_correspondence_map(source_points, source_normals, source_mask,
+                                   rt_cam, target_points, target_normals,
+                                   target_mask, kcam, corresp_map,
+                                   distance_thresh, normal_angle_thesh):
+        """include/slamtb/matching.hpp:11-21. ``corresp_map`` [H, W] int64 is
+        min-merged in place with ``(index << 32) | float_bits(distance)``."""
+        what = "CorrespondenceMap.compute_correspondence_map"
+        require_gpu(source_points, source_normals, source_mask, rt_cam,
+                    target_points, target_normals, target_mask, kcam,
+                    corresp_map, what=what)
+        require_contiguous_out(corresp_map, what=what)
+        if corresp_map.dtype != torch.int64:
+            raise RuntimeError(f"{what}: corresp_map must be int64")
+        dt = source_points.dtype
+        sp = _in(source_points, dt).view(-1, 3)
+        sn = _in(source_normals, dt).view(-1, 3)
+        sm = _mask(source_mask).view(-1)
+        tp = _in(target_points, dt)
+        tn = _in(target_normals, dt)
+        tm = _mask(target_mask)
+        H, W = tm.shape[0], tm.shape[1]
+        if corresp_map.shape[0] != H or corresp_map.shape[1] != W:
+            raise RuntimeError(f"{what}: corresp_map must be [{H}, {W}]")
+        with torch.cuda.device(sp.device):
+            check(lib().stb_compute_correspondence_map(
+                dtype_code(sp), ptr(sp), ptr(sn), ptr(sm), sm.numel(),
+                sp.size(0), ptr(_rt(rt_cam, dt)), _rt(rt_cam, dt).size(0),
+                ptr(tp), ptr(tn), ptr(tm), H, W, ptr(_kcam(kcam, dt)),
+                ptr(corresp_map), float(distance_thresh),
+                float(normal_angle_thesh), stream()), what)
+
+
+class ICPJacobian:
+    @staticmethod
+    def estimate_geometric(src_points, src_mask, rt_cam, tgt_points,
+                           tgt_normals, kcam, corresp_map, JtJ_partial,
+                           Jr_partial, squared_residual):
+        """include/slamtb/icp_jacobian.hpp:72-77; returns the match count."""
+        what = "ICPJacobian.estimate_geometric"
+        require_gpu(src_points, src_mask, rt_cam, tgt_points, tgt_normals,
+                    kcam, corresp_map, JtJ_partial, Jr_partial,
+                    squared_residual, what=what)
+        require_contiguous_out(JtJ_partial, Jr_partial, squared_residual,
+                               what=what)
+        dt = src_points.dtype
+        sp = _in(src_points, dt).view(-1, 3)
+        n = sp.size(0)
+        for t, shape in ((JtJ_partial, (n, 6, 6)), (Jr_partial, (n, 6)),
+                         (squared_residual, (n,))):
+            if tuple(t.shape) != shape or t.dtype != dt:
+                raise RuntimeError(f"{what}: expected {shape} {dt} output")
+        if tgt_points.dtype != torch.float32 or tgt_normals.dtype != torch.float32:
+            raise RuntimeError(f"{what}: target maps must be float32 "
+                               "(src/icp_jacobian_geometric.cu:22-23)")
+        sm = _mask(src_mask).view(-1)
+        cmap = _in(corresp_map)
+        H, W = cmap.shape
+        count = torch.empty(1, dtype=torch.int32, device=sp.device)
+        rt = _rt(rt_cam, dt)
+        with torch.cuda.device(sp.device):
+            check(lib().stb_icp_estimate_geometric(
+                dtype_code(sp), ptr(sp), ptr(sm), n, ptr(rt), rt.size(0),
+                ptr(_in(tgt_points)), ptr(_in(tgt_normals)), H, W,
+                ptr(_kcam(kcam, dt)), ptr(cmap), ptr(JtJ_partial),
+                ptr(Jr_partial), ptr(squared_residual), ptr(count), stream()),
+                what)
+        return int(count.item())
+
+    @staticmethod
+    def get_information_matrix(src_points, src_mask, rt_cam, tgt_points, kcam,
+                               corresp_map, GtG_partial):
+        """include/slamtb/icp_jacobian.hpp:79-85."""
+        what = "ICPJacobian.get_information_matrix"
+        require_gpu(src_points, src_mask, rt_cam, tgt_points, kcam,
+                    corresp_map, GtG_partial, what=what)
+        require_contiguous_out(GtG_partial, what=what)
+        dt = src_points.dtype
+        sp = _in(src_points, dt).view(-1, 3)
+        n = sp.size(0)
+        if tuple(GtG_partial.shape) != (n, 6, 6) or GtG_partial.dtype != dt:
+            raise RuntimeError(f"{what}: expected ({n}, 6, 6) {dt} output")
+        sm = _mask(src_mask).view(-1)
+        cmap = _in(corresp_map)
+        H, W = cmap.shape
+        rt = _rt(rt_cam, dt)
+        with torch.cuda.device(sp.device):
+            check(lib().stb_icp_get_information_matrix(
+                dtype_code(sp), ptr(sp), ptr(sm), n, ptr(rt), rt.size(0),
+                ptr(_in(tgt_points, torch.float32)), H, W,
+                ptr(_kcam(kcam, dt)), ptr(cmap), ptr(GtG_partial), stream()),
+                what)
+
+    @staticmethod
+    def estimate_feature(src_points, src_feats, src_mask, rt_cam, tgt_feats,
+                         kcam, corresp_map, residual_thresh, JtJ_partial,
+                         Jr_partial, squared_residuals):
+        """include/slamtb/icp_jacobian.hpp:140-146; returns the match count."""
+        what = "ICPJacobian.estimate_feature"
+        require_gpu(src_points, src_feats, src_mask, rt_cam, tgt_feats, kcam,
+                    corresp_map, JtJ_partial, Jr_partial, squared_residuals,
+                    what=what)
+        require_contiguous_out(JtJ_partial, Jr_partial, squared_residuals,
+                               what=what)
+        dt = src_points.dtype
+        sp = _in(src_points, dt).view(-1, 3)
+        n = sp.size(0)
+        tf = _in(tgt_feats, dt)
+        C, H, W = tf.shape
+        sf = _in(src_feats, dt).view(C, n)
+        for t, shape in ((JtJ_partial, (n, 6, 6)), (Jr_partial, (n, 6)),
+                         (squared_residuals, (n,))):
+            if tuple(t.shape) != shape or t.dtype != dt:
+                raise RuntimeError(f"{what}: expected {shape} {dt} output")
+        sm = _mask(src_mask).view(-1)
+        cmap = _in(corresp_map)
+        count = torch.empty(1, dtype=torch.int32, device=sp.device)
+        rt = _rt(rt_cam, dt)
+        with torch.cuda.device(sp.device):
+            check(lib().stb_icp_estimate_feature(
+                dtype_code(sp), ptr(sp), ptr(sf), ptr(sm), n, ptr(rt),
+                rt.size(0), ptr(tf), C, H, W, ptr(_kcam(kcam, dt)), ptr(cmap),
+                float(residual_thresh), ptr(JtJ_partial), ptr(Jr_partial),
+                ptr(squared_residuals), ptr(count), stream()), what)
+        return int(count.item())
+
+    @staticmethod
+    def estimate_feature_so3(*args):
+        """include/slamtb/icp_jacobian.hpp:201-207. The reference's only
+        caller passes 16 arguments to this 11-parameter binding
+        (slamtb/registration/icp.py:71-76), so the path cannot run there
+        either; it is not provided."""
+        raise NotImplementedError(
+            "estimate_feature_so3 is not provided: the reference's SO3 path "
+            "is unreachable (wrong call arity in icp.py:71-76)")
+
+
+class Processing:
+    @staticmethod
+    def bilateral_depth_filter(input, mask, result, filter_width, sigma_color,
+                               sigma_space, depth_scale):
+        """src/bilateral_depth_filter.cu:79-110 (positional parameters as the
+        binding passes them)."""
+        what = "Processing.bilateral_depth_filter"
+        require_gpu(input, mask, result, what=what)
+        require_contiguous_out(result, what=what)
+        inp = _in(input)
+        if result.dtype != inp.dtype or result.shape != inp.shape:
+            raise RuntimeError(f"{what}: result must match input")
+        m = _mask(mask)
+        H, W = inp.shape
+        with torch.cuda.device(inp.device):
+            check(lib().stb_bilateral_depth_filter(
+                dtype_code(inp), ptr(inp), ptr(m), ptr(result), H, W,
+                int(filter_width), float(sigma_color), float(sigma_space),
+                float(depth_scale), stream()), what)
+        return result
+
+    @staticmethod
+    def estimate_normals(xyz, mask, out, method):
+        """src/normals.cu:190-208 (CentralDifferences only)."""
+        what = "Processing.estimate_normals"
+        require_gpu(xyz, mask, out, what=what)
+        require_contiguous_out(out, what=what)
+        x = _in(xyz)
+        if out.dtype != x.dtype or out.shape != x.shape:
+            raise RuntimeError(f"{what}: out must match xyz")
+        m = _mask(mask)
+        H, W = x.shape[:2]
+        with torch.cuda.device(x.device):
+            check(lib().stb_estimate_normals(
+                dtype_code(x), ptr(x), ptr(m), ptr(out), H, W, int(method),
+                stream()), what)
+
+    @staticmethod
+    def downsample_xyz(src, mask, scale, dst, normalize, method):
+        """src/downsample.cu:75-98."""
+        what = "Processing.downsample_xyz"
+        require_gpu(src, mask, dst, what=what)
+        require_contiguous_out(dst, what=what)
+        s = _in(src)
+        m = _mask(mask)
+        H, W = s.shape[:2]
+        with torch.cuda.device(s.device):
+            check(lib().stb_downsample_xyz(
+                dtype_code(s), ptr(s), ptr(m), H, W, float(scale), ptr(dst),
+                dst.size(0), dst.size(1), int(bool(normalize)), int(method),
+                stream()), what)
+
+    @staticmethod
+    def downsample_mask(mask, scale, dst):
+        """src/downsample.cu:119-131."""
+        what = "Processing.downsample_mask"
+        require_gpu(mask, dst, what=what)
+        require_contiguous_out(dst, what=what)
+        m = _mask(mask)
+        H, W = m.shape
+        with torch.cuda.device(m.device):
+            check(lib().stb_downsample_mask(
+                ptr(m), H, W, float(scale), ptr(dst), dst.size(0), dst.size(1),
+                stream()), what)
+
+    @staticmethod
+    def downsample_features(feat, scale, dst):
+        """src/downsample.cu:171-191."""
+        what = "Processing.downsample_features"
+        require_gpu(feat, dst, what=what)
+        require_contiguous_out(dst, what=what)
+        f = _in(feat)
+        C, H, W = f.shape
+        with torch.cuda.device(f.device):
+            check(lib().stb_downsample_features(
+                dtype_code(f), ptr(f), C, H, W, float(scale), ptr(dst),
+                dst.size(1), dst.size(2), stream()), what)
+
+    @staticmethod
+    def erode_mask(in_mask, out_mask):
+        """src/erode_mask.cu:42-54."""
+        what = "Processing.erode_mask"
+        require_gpu(in_mask, out_mask, what=what)
+        require_contiguous_out(out_mask, what=what)
+        m = _mask(in_mask)
+        H, W = m.shape
+        with torch.cuda.device(m.device):
+            check(lib().stb_erode_mask(ptr(m), ptr(out_mask), H, W, stream()),
+                  what)
+
+
+class ExpRtToMatrixOp:
+    @staticmethod
+    def forward(exp_rt, matrix):
+        """src/exp_rt_to_matrix.cu:41-52: [B,6] -> [B,3,4]."""
+        what = "ExpRtToMatrixOp.forward"
+        require_gpu(exp_rt, matrix, what=what)
+        require_contiguous_out(matrix, what=what)
+        x = _in(exp_rt).view(-1, 6)
+        with torch.cuda.device(x.device):
+            check(lib().stb_exp_rt_to_matrix_forward(
+                dtype_code(x), ptr(x), x.size(0), ptr(matrix), stream()), what)
+
+    @staticmethod
+    def backward(d_matrix, exp_rt, y_matrix, d_exp_rt):
+        """src/exp_rt_to_matrix.cu:104-121."""
+        what = "ExpRtToMatrixOp.backward"
+        require_gpu(d_matrix, exp_rt, y_matrix, d_exp_rt, what=what)
+        require_contiguous_out(d_exp_rt, what=what)
+        x = _in(exp_rt).view(-1, 6)
+        dt = x.dtype
+        with torch.cuda.device(x.device):
+            check(lib().stb_exp_rt_to_matrix_backward(
+                dtype_code(x), ptr(_in(d_matrix, dt)), ptr(x),
+                ptr(_in(y_matrix, dt)), x.size(0), ptr(d_exp_rt), stream()),
+                what)
+
+
+class MatrixToExpRtOp:
+    @staticmethod
+    def forward(matrix, exp_rt):
+        """src/matrix_to_exp_rt.cu:38-50: [B,3,4] -> [B,6]."""
+        what = "MatrixToExpRtOp.forward"
+        require_gpu(matrix, exp_rt, what=what)
+        require_contiguous_out(exp_rt, what=what)
+        m = _in(matrix)
+        if m.dim() == 2:
+            m = m.unsqueeze(0)
+        m = m[:, :3, :4].contiguous()
+        with torch.cuda.device(m.device):
+            check(lib().stb_matrix_to_exp_rt_forward(
+                dtype_code(m), ptr(m), m.size(0), ptr(exp_rt), stream()), what)
+
+
+__all__ = ["CorrespondenceMap", "ICPJacobian", "Processing",
+           "EstimateNormalsMethod", "DownsampleXYZMethod", "ExpRtToMatrixOp",
+           "MatrixToExpRtOp", "_lib"]

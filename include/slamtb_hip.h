@@ -1,0 +1,275 @@
+/*
+ * libslamtb_hip.so -- C ABI of the MI355X (gfx950) ICP Gauss-Newton step.
+ *
+ * Replaces the CUDA half of the reference's pybind11 module `_cslamtb`
+ * (src/_cslamtb.cpp:22-53). Conventions of every entry point:
+ *   - all array pointers are device pointers to C-contiguous buffers that the
+ *     caller owns; the library allocates nothing and keeps no state;
+ *   - `stream` is a hipStream_t (NULL = the default stream); work is only
+ *     enqueued, the library never synchronises the device;
+ *   - the return value is STB_OK (0) or an error code, and stb_last_error()
+ *     gives the message of the last failure on the calling thread;
+ *   - 3x4 / 4x4 matrices are row-major; `rt_rows` is 3 or 4 (only the first
+ *     three rows are read), `kcam` is a row-major 3x3 matrix.
+ *   - masks are one byte per element (torch.bool), non-zero = valid.
+ */
+#ifndef SLAMTB_HIP_H_
+#define SLAMTB_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum stb_status {
+  STB_OK = 0,
+  STB_ERR_INVALID = 1, /* bad argument: shape, dtype, null pointer */
+  STB_ERR_LAUNCH = 2,  /* a kernel launch failed */
+  STB_ERR_UNSUPPORTED = 3
+};
+
+enum stb_dtype {
+  STB_F32 = 0,
+  STB_F64 = 1,
+  STB_BF16 = 2,
+  STB_F16 = 3,
+  STB_I32 = 4,
+  STB_U8 = 5,
+  STB_I64 = 6,
+  STB_I16 = 7
+};
+
+/* Last error message of the calling thread ("" if none). */
+const char* stb_last_error(void);
+/* ABI version, bumped on any signature change. */
+int stb_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in entry points (same argument meaning as the reference bindings).   */
+/* ------------------------------------------------------------------------ */
+
+/* CorrespondenceMap.compute_correspondence_map
+ * (include/slamtb/matching.hpp:11-21, src/correspondence_map.cu:64-102).
+ * dtype: STB_F32 or STB_F64 for points, normals, rt_cam and kcam.
+ * src_mask is applied only when src_mask_len == n_src (else ignored, as in
+ * src/correspondence_map.cu:40-41). corresp_map [height*width] int64 holds
+ * (source_index << 32) | float_bits(distance) and is min-merged in place;
+ * among equal distances the lowest source index is kept. */
+int stb_compute_correspondence_map(
+    int dtype, const void* src_points, const void* src_normals,
+    const uint8_t* src_mask, int64_t src_mask_len, int64_t n_src,
+    const void* rt_cam, int rt_rows, const void* tgt_points,
+    const void* tgt_normals, const uint8_t* tgt_mask, int32_t height,
+    int32_t width, const void* kcam, int64_t* corresp_map,
+    double distance_thresh, double normal_angle_thresh, void* stream);
+
+/* ICPJacobian.estimate_geometric (include/slamtb/icp_jacobian.hpp:72-77,
+ * src/icp_jacobian_geometric.cu:163-214). Target maps are float32 for both
+ * dtypes (src/icp_jacobian_geometric.cu:22-23). Writes per-pixel
+ * JtJ_partial [n,6,6], Jr_partial [n,6], squared_residual [n] in `dtype`;
+ * *match_count (device int32) is zeroed and then counts the matches. */
+int stb_icp_estimate_geometric(
+    int dtype, const void* src_points, const uint8_t* src_mask, int64_t n_src,
+    const void* rt_cam, int rt_rows, const float* tgt_points,
+    const float* tgt_normals, int32_t height, int32_t width, const void* kcam,
+    const int64_t* corresp_map, void* JtJ_partial, void* Jr_partial,
+    void* squared_residual, int32_t* match_count, void* stream);
+
+/* ICPJacobian.get_information_matrix (include/slamtb/icp_jacobian.hpp:79-85,
+ * src/icp_jacobian_geometric.cu:216-249). GtG_partial [n,6,6] in `dtype`,
+ * parameter order [rotation, translation]. */
+int stb_icp_get_information_matrix(
+    int dtype, const void* src_points, const uint8_t* src_mask, int64_t n_src,
+    const void* rt_cam, int rt_rows, const float* tgt_points, int32_t height,
+    int32_t width, const void* kcam, const int64_t* corresp_map,
+    void* GtG_partial, void* stream);
+
+/* ICPJacobian.estimate_feature (include/slamtb/icp_jacobian.hpp:140-146,
+ * src/icp_jacobian_feature.cu:113-169). src_feats [channels, n_src],
+ * tgt_feats [channels, height, width], both in `dtype`. As in the reference
+ * the per-pixel J keeps only the last channel while the squared residual
+ * sums every channel; pixels above residual_thresh^2 are zeroed, uncounted. */
+int stb_icp_estimate_feature(
+    int dtype, const void* src_points, const void* src_feats,
+    const uint8_t* src_mask, int64_t n_src, const void* rt_cam, int rt_rows,
+    const void* tgt_feats, int32_t channels, int32_t height, int32_t width,
+    const void* kcam, const int64_t* corresp_map, float residual_thresh,
+    void* JtJ_partial, void* Jr_partial, void* squared_residual,
+    int32_t* match_count, void* stream);
+
+/* Processing.bilateral_depth_filter (src/processing.cpp:9,
+ * src/bilateral_depth_filter.cu:79-110). dtype: STB_F32, STB_F64, STB_I32,
+ * STB_I16 or STB_U8 for input and result [height, width]. */
+int stb_bilateral_depth_filter(int dtype, const void* input,
+                               const uint8_t* mask, void* result,
+                               int32_t height, int32_t width,
+                               int32_t filter_width, float sigma_color,
+                               float sigma_space, float depth_scale,
+                               void* stream);
+
+/* Processing.estimate_normals (src/processing.cpp:10, src/normals.cu:190-208).
+ * method 0 = CentralDifferences, 1 = Average8 (unsupported here). */
+int stb_estimate_normals(int dtype, const void* xyz, const uint8_t* mask,
+                         void* out_normals, int32_t height, int32_t width,
+                         int method, void* stream);
+
+/* Processing.downsample_xyz (src/processing.cpp:11, src/downsample.cu:75-98).
+ * `normalize` is accepted and ignored, as in the reference; method 0 =
+ * Nearest. dst is [dst_height, dst_width, 3]. */
+int stb_downsample_xyz(int dtype, const void* src, const uint8_t* mask,
+                       int32_t height, int32_t width, float scale, void* dst,
+                       int32_t dst_height, int32_t dst_width, int normalize,
+                       int method, void* stream);
+
+/* Processing.downsample_mask (src/processing.cpp:12, src/downsample.cu:119-131). */
+int stb_downsample_mask(const uint8_t* mask, int32_t height, int32_t width,
+                        float scale, uint8_t* dst, int32_t dst_height,
+                        int32_t dst_width, void* stream);
+
+/* Processing.downsample_features (src/processing.cpp:13,
+ * src/downsample.cu:171-191). src [channels, height, width]. */
+int stb_downsample_features(int dtype, const void* src, int32_t channels,
+                            int32_t height, int32_t width, float scale,
+                            void* dst, int32_t dst_height, int32_t dst_width,
+                            void* stream);
+
+/* Processing.erode_mask (src/processing.cpp:14, src/erode_mask.cu:42-54). */
+int stb_erode_mask(const uint8_t* in_mask, uint8_t* out_mask, int32_t height,
+                   int32_t width, void* stream);
+
+/* Separable normalised 3x3 Gaussian blur with reflect borders over
+ * [channels, height, width]; stands in for the kornia GaussianBlur2d call of
+ * feature_pyramid (slamtb/processing.py:148-153). */
+int stb_gaussian_blur3x3(int dtype, const void* src, int32_t channels,
+                         int32_t height, int32_t width, double sigma,
+                         void* dst, void* stream);
+
+/* depth_image_to_uvz + KCamera.backproject (slamtb/_utils.py:11-37,
+ * slamtb/camera.py:127-149): points [height, width, 3] float32 from a depth
+ * image of type `depth_dtype` (STB_F32, STB_F64, STB_I32, STB_I16, STB_U8). */
+int stb_depth_to_points(int depth_dtype, const void* depth, int32_t height,
+                        int32_t width, float fx, float fy, float cx, float cy,
+                        float depth_scale, float depth_bias, float* points,
+                        void* stream);
+
+/* ExpRtToMatrixOp.forward (src/se3.cpp:9, src/exp_rt_to_matrix.cu:41-52):
+ * exp_rt [batch, 6] = [t; w] -> matrix [batch, 3, 4] = [R(w) | t]. */
+int stb_exp_rt_to_matrix_forward(int dtype, const void* exp_rt, int64_t batch,
+                                 void* matrix, void* stream);
+
+/* ExpRtToMatrixOp.backward (src/se3.cpp:10, src/exp_rt_to_matrix.cu:104-121). */
+int stb_exp_rt_to_matrix_backward(int dtype, const void* d_matrix,
+                                  const void* exp_rt, const void* y_matrix,
+                                  int64_t batch, void* d_exp_rt, void* stream);
+
+/* MatrixToExpRtOp.forward (src/se3.cpp:14, src/matrix_to_exp_rt.cu:38-50). */
+int stb_matrix_to_exp_rt_forward(int dtype, const void* matrix, int64_t batch,
+                                 void* exp_rt, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused Gauss-Newton path used by slamtb.registration.ICPOdometry.          */
+/* It replaces the per-iteration sequence of icp.py:210-256 (two maps, two   */
+/* per-pixel partial kernels, torch sums, host Cholesky and SE3 update)      */
+/* with two kernels per iteration and no host round trip.                    */
+/* ------------------------------------------------------------------------ */
+
+/* One pyramid level of one frame pair. Storage dtype (STB_F32, STB_BF16,
+ * STB_F16) applies to points, normals and features; masks are bytes.
+ * Source arrays are [n_src, 3] / [channels, n_src]; target arrays are
+ * [height, width, 3] / [channels, height, width]. Set src_feats/tgt_feats to
+ * NULL (or channels to 0) for a geometric-only level. */
+typedef struct stb_icp_level {
+  const void* src_points;
+  const void* src_normals;
+  const uint8_t* src_mask;
+  const void* src_feats;
+  const void* tgt_points;
+  const void* tgt_normals;
+  const uint8_t* tgt_mask;
+  const void* tgt_feats;
+  int64_t n_src;
+  int32_t height;
+  int32_t width;
+  int32_t channels;
+  int32_t storage;
+  float fx, fy, cx, cy;
+} stb_icp_level;
+
+/* Options of ICPOdometry (slamtb/registration/icp.py:137-150). */
+typedef struct stb_icp_params {
+  double geom_weight;
+  double feat_weight;
+  double distance_thresh;
+  double normal_angle_thresh;
+  float feat_residual_thresh;
+  int32_t use_geom;     /* geom_weight > 0 */
+  int32_t use_feat;     /* feat_weight > 0 and both feature maps given */
+  int32_t feat_mode;    /* 0: last channel's J (reference), 1: all channels */
+} stb_icp_params;
+
+/* Device-resident result, at offset 0 of the workspace. Fields mirror
+ * RegistrationResult (slamtb/registration/result.py:9-46). */
+typedef struct stb_icp_state {
+  double transform[16];   /* current pose, row-major 4x4 */
+  double best_transform[16];
+  double best_jtj[36];    /* combined JtJ of the last accepted iteration */
+  double information[36]; /* G^T G at the final pose */
+  double best_residual;   /* residual / match_count */
+  double best_match_count;
+  double last_jtj[36];    /* combined system of the most recent iteration */
+  double last_jr[6];
+  double last_geom_r2, last_feat_r2, last_geom_count, last_feat_count;
+  float rt[12];           /* float copy of the pose rows (transform.to(f32)) */
+  float nm[9];            /* inverse-transpose of rt's rotation */
+  int32_t stopped;        /* 1 after a failed Cholesky (the loop's `break`) */
+  int32_t iterations;     /* accepted iterations */
+  int32_t ticket;         /* block-arrival counter of the reduction kernels */
+  int32_t pad_;
+} stb_icp_state;
+
+/* Bytes of workspace needed for a level of n_src source and height*width
+ * target pixels (state, two correspondence maps, block partials). */
+size_t stb_icp_workspace_bytes(int64_t n_src, int32_t height, int32_t width);
+
+/* Resets the state to the 4x4 row-major FP64 pose `transform0` (device
+ * pointer, NULL = identity) and clears both correspondence maps. With
+ * `chain` != 0 the start pose is instead the workspace's own best_transform
+ * when its previous run accepted at least one iteration, else identity: the
+ * hand-over between pyramid levels of MultiscaleRegistration.estimate_frame
+ * (slamtb/registration/multiscale.py:52-65). */
+int stb_icp_begin(void* workspace, size_t workspace_bytes,
+                  const double* transform0, int32_t chain, int64_t n_src,
+                  int32_t height, int32_t width, void* stream);
+
+/* Enqueues `num_iters` Gauss-Newton iterations. Each iteration runs a
+ * correspondence kernel and a fused Jacobian/reduction kernel whose last
+ * block sums the per-block partials in FP64 in a fixed order, solves the
+ * 6x6 system by Cholesky in FP64 and left-multiplies exp_rt_to_matrix(x)
+ * into the pose. A failed factorisation sets `stopped`, after which the
+ * remaining iterations do nothing. `first_iter` numbers the iterations for
+ * the map double-buffering (pass the count already enqueued since begin). */
+int stb_icp_iterate(const stb_icp_level* level, const stb_icp_params* params,
+                    void* workspace, size_t workspace_bytes, int32_t first_iter,
+                    int32_t num_iters, void* stream);
+
+/* One linearisation at the current pose without a pose update: fills
+ * last_jtj / last_jr / last_*_r2 / last_*_count with the reduced system of
+ * the active terms (weights and the feature term's sign applied as in the
+ * update). The fused counterpart of one _JacobianStep call per term
+ * (slamtb/registration/icp.py:30-82). */
+int stb_icp_linearize(const stb_icp_level* level, const stb_icp_params* params,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
+/* Fills state.information with sum of G^T G at the current pose
+ * (src/icp_jacobian_geometric.cu:88-159 summed). */
+int stb_icp_information(const stb_icp_level* level,
+                        const stb_icp_params* params, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLAMTB_HIP_H_ */
