@@ -600,6 +600,7 @@ __device__ inline void gn_finalize(stb_icp_state* st, const double* tot,
   st->best_residual = (fw * tot[28] + gw * tot[27]) / count;
   st->best_match_count = count;
   st->iterations += 1;
+  st->map_sel ^= 1;
   set_pose_floats(st);
 }
 
@@ -632,7 +633,19 @@ __global__ void icp_begin_kernel(stb_icp_state* st,
   st->stopped = 0;
   st->iterations = 0;
   st->ticket = 0;
+  st->map_sel = 0;
   set_pose_floats(st);
+}
+
+// Clears the map that state.map_sel does not name (the spare one).
+__global__ void __launch_bounds__(kBlock)
+    fill_spare_map_kernel(const stb_icp_state* __restrict__ st,
+                          unsigned long long* __restrict__ map0,
+                          unsigned long long* __restrict__ map1, int64_t n) {
+  unsigned long long* __restrict__ p = st->map_sel ? map0 : map1;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = kKeySentinel;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -650,10 +663,16 @@ __global__ void __launch_bounds__(kBlock)
 template <typename S>
 __global__ void __launch_bounds__(kBlock)
     corresp_fused_kernel(stb_icp_level L, const stb_icp_state* __restrict__ st,
-                         unsigned long long* __restrict__ map, float dthr2,
-                         float cstar, int ignore_stop) {
+                         unsigned long long* __restrict__ map0,
+                         unsigned long long* __restrict__ map1, float dthr2,
+                         float cstar, int fixed_map) {
 #pragma clang fp contract(off)
-  if (!ignore_stop && st->stopped) return;
+  // fixed_map: a one-off pass on the spare map that ignores `stopped`
+  // (information, linearisation); otherwise the GN loop's map, the one
+  // state.map_sel names.
+  if (!fixed_map && st->stopped) return;
+  unsigned long long* __restrict__ map =
+      (st->map_sel ^ fixed_map) ? map1 : map0;
   float m[12], nm[9];
 #pragma unroll
   for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
@@ -703,19 +722,26 @@ struct TermWeights {
 template <typename S, bool GEOM, bool FEAT>
 __global__ void __launch_bounds__(kBlock)
     jacobian_fused_kernel(stb_icp_level L, stb_icp_state* __restrict__ st,
-                          const unsigned long long* __restrict__ map,
-                          unsigned long long* __restrict__ next_map,
-                          int64_t next_len, double* __restrict__ partials,
-                          TermWeights tw, int do_solve, int ignore_stop) {
+                          unsigned long long* __restrict__ map0,
+                          unsigned long long* __restrict__ map1,
+                          int64_t map_len, double* __restrict__ partials,
+                          TermWeights tw, int do_solve, int fixed_map) {
 #pragma clang fp contract(off)
   __shared__ double sh[kShared];
   __shared__ double tot[kSlots];
   __shared__ int s_last;
-  if (!ignore_stop && st->stopped) return;
+  // GN loop (fixed_map == 0): read the map state.map_sel names and clear the
+  // other one for the next iteration; the solve flips map_sel. Because the
+  // choice lives on the device, a captured sequence of any length replays
+  // correctly. fixed_map != 0: a one-off pass on the spare map.
+  if (!fixed_map && st->stopped) return;
+  const int sel = st->map_sel ^ (fixed_map ? 1 : 0);
+  const unsigned long long* __restrict__ map = sel ? map1 : map0;
   const int64_t gsize = int64_t(gridDim.x) * blockDim.x;
   const int64_t gtid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (next_map) {
-    for (int64_t i = gtid; i < next_len; i += gsize) next_map[i] = kKeySentinel;
+  if (!fixed_map) {
+    unsigned long long* __restrict__ next_map = sel ? map0 : map1;
+    for (int64_t i = gtid; i < map_len; i += gsize) next_map[i] = kKeySentinel;
   }
   float m[12];
 #pragma unroll
@@ -794,9 +820,11 @@ __global__ void __launch_bounds__(kBlock)
 template <typename S>
 __global__ void __launch_bounds__(kBlock)
     information_fused_kernel(stb_icp_level L, stb_icp_state* __restrict__ st,
-                             const unsigned long long* __restrict__ map,
+                             const unsigned long long* __restrict__ map0,
+                             const unsigned long long* __restrict__ map1,
                              double* __restrict__ partials) {
 #pragma clang fp contract(off)
+  const unsigned long long* __restrict__ map = st->map_sel ? map0 : map1;
   __shared__ double sh[kShared];
   __shared__ double tot[kSlots];
   __shared__ int s_last;
@@ -923,81 +951,63 @@ int check_level(const stb_icp_level* L, bool need_feats) {
   return STB_OK;
 }
 
+// Phases of one GN iteration. kCorrespond and kUpdate are the two kernels of
+// the loop (on the map state.map_sel names); kLinearize and kInformation are
+// one-off passes on the other map, which the loop clears before its next use.
+enum Phase { kCorrespond, kUpdate, kLinearize, kInformation };
+
+template <typename S, bool GEOM, bool FEAT>
+void launch_jacobian(const stb_icp_level& L, const Workspace& w,
+                     const TermWeights& tw, int do_solve, int fixed_map,
+                     hipStream_t stream) {
+  hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT>),
+                     dim3(fused_grid(L.n_src)), dim3(kBlock), 0, stream, L,
+                     w.st, w.map[0], w.map[1], w.map_len, w.partials, tw,
+                     do_solve, fixed_map);
+}
+
+template <typename S>
+void launch_phase(const stb_icp_level& L, const stb_icp_params& P,
+                  const Workspace& w, Phase phase, hipStream_t stream) {
+  const float dthr2 = float(P.distance_thresh * P.distance_thresh);
+  const float cstar = angle_gate_cos_f32(P.normal_angle_thresh);
+  const int gA = grid_for(L.n_src, kBlock, 4096);
+  const int fixed = (phase == kLinearize || phase == kInformation) ? 1 : 0;
+  if (fixed) {
+    hipLaunchKernelGGL(fill_spare_map_kernel,
+                       dim3(grid_for(w.map_len, kBlock, 2048)), dim3(kBlock), 0,
+                       stream, w.st, w.map[0], w.map[1], w.map_len);
+  }
+  if (phase != kUpdate) {
+    hipLaunchKernelGGL(corresp_fused_kernel<S>, dim3(gA), dim3(kBlock), 0,
+                       stream, L, w.st, w.map[0], w.map[1], dthr2, cstar,
+                       fixed);
+  }
+  if (phase == kCorrespond) return;
+  if (phase == kInformation) {
+    hipLaunchKernelGGL(information_fused_kernel<S>, dim3(fused_grid(L.n_src)),
+                       dim3(kBlock), 0, stream, L, w.st, w.map[0], w.map[1],
+                       w.partials);
+    return;
+  }
+  const TermWeights tw = make_weights(P);
+  const int solve = phase == kUpdate ? 1 : 0;
+  if (P.use_geom && P.use_feat)
+    launch_jacobian<S, true, true>(L, w, tw, solve, fixed, stream);
+  else if (P.use_geom)
+    launch_jacobian<S, true, false>(L, w, tw, solve, fixed, stream);
+  else
+    launch_jacobian<S, false, true>(L, w, tw, solve, fixed, stream);
+}
+
 template <typename S>
 int launch_iterations(const stb_icp_level& L, const stb_icp_params& P,
-                      const Workspace& w, int first, int count,
-                      hipStream_t stream) {
-  const float dthr2 = float(P.distance_thresh * P.distance_thresh);
-  const float cstar = angle_gate_cos_f32(P.normal_angle_thresh);
-  const bool geom = P.use_geom != 0, feat = P.use_feat != 0;
-  const TermWeights tw = make_weights(P);
-  const int gA = grid_for(L.n_src, kBlock, 4096);
-  const int gB = fused_grid(L.n_src);
-  for (int it = first; it < first + count; ++it) {
-    unsigned long long* cur = w.map[it & 1];
-    unsigned long long* nxt = w.map[(it + 1) & 1];
-    hipLaunchKernelGGL(corresp_fused_kernel<S>, dim3(gA), dim3(kBlock), 0,
-                       stream, L, w.st, cur, dthr2, cstar, 0);
-    if (geom && feat)
-      hipLaunchKernelGGL((jacobian_fused_kernel<S, true, true>), dim3(gB),
-                         dim3(kBlock), 0, stream, L, w.st, cur, nxt, w.map_len,
-                         w.partials, tw, 1, 0);
-    else if (geom)
-      hipLaunchKernelGGL((jacobian_fused_kernel<S, true, false>), dim3(gB),
-                         dim3(kBlock), 0, stream, L, w.st, cur, nxt, w.map_len,
-                         w.partials, tw, 1, 0);
-    else
-      hipLaunchKernelGGL((jacobian_fused_kernel<S, false, true>), dim3(gB),
-                         dim3(kBlock), 0, stream, L, w.st, cur, nxt, w.map_len,
-                         w.partials, tw, 1, 0);
+                      const Workspace& w, int count, hipStream_t stream) {
+  for (int it = 0; it < count; ++it) {
+    launch_phase<S>(L, P, w, kCorrespond, stream);
+    launch_phase<S>(L, P, w, kUpdate, stream);
   }
   return check_launch("stb_icp_iterate");
-}
-
-template <typename S>
-int launch_linearize(const stb_icp_level& L, const stb_icp_params& P,
-                     const Workspace& w, hipStream_t stream) {
-  const float dthr2 = float(P.distance_thresh * P.distance_thresh);
-  const float cstar = angle_gate_cos_f32(P.normal_angle_thresh);
-  TermWeights tw = make_weights(P);
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(w.map_len, kBlock, 2048)),
-                     dim3(kBlock), 0, stream, w.map[0], w.map_len,
-                     (unsigned long long)kKeySentinel);
-  hipLaunchKernelGGL(corresp_fused_kernel<S>,
-                     dim3(grid_for(L.n_src, kBlock, 4096)), dim3(kBlock), 0,
-                     stream, L, w.st, w.map[0], dthr2, cstar, 1);
-  const bool geom = P.use_geom != 0, feat = P.use_feat != 0;
-  const int gB = fused_grid(L.n_src);
-  unsigned long long* none = nullptr;
-  if (geom && feat)
-    hipLaunchKernelGGL((jacobian_fused_kernel<S, true, true>), dim3(gB),
-                       dim3(kBlock), 0, stream, L, w.st, w.map[0], none,
-                       int64_t(0), w.partials, tw, 0, 1);
-  else if (geom)
-    hipLaunchKernelGGL((jacobian_fused_kernel<S, true, false>), dim3(gB),
-                       dim3(kBlock), 0, stream, L, w.st, w.map[0], none,
-                       int64_t(0), w.partials, tw, 0, 1);
-  else
-    hipLaunchKernelGGL((jacobian_fused_kernel<S, false, true>), dim3(gB),
-                       dim3(kBlock), 0, stream, L, w.st, w.map[0], none,
-                       int64_t(0), w.partials, tw, 0, 1);
-  return check_launch("stb_icp_linearize");
-}
-
-template <typename S>
-int launch_information(const stb_icp_level& L, const stb_icp_params& P,
-                       const Workspace& w, hipStream_t stream) {
-  const float dthr2 = float(P.distance_thresh * P.distance_thresh);
-  const float cstar = angle_gate_cos_f32(P.normal_angle_thresh);
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(w.map_len, kBlock, 2048)),
-                     dim3(kBlock), 0, stream, w.map[0], w.map_len,
-                     (unsigned long long)kKeySentinel);
-  hipLaunchKernelGGL(corresp_fused_kernel<S>,
-                     dim3(grid_for(L.n_src, kBlock, 4096)), dim3(kBlock), 0,
-                     stream, L, w.st, w.map[0], dthr2, cstar, 1);
-  hipLaunchKernelGGL(information_fused_kernel<S>, dim3(fused_grid(L.n_src)),
-                     dim3(kBlock), 0, stream, L, w.st, w.map[0], w.partials);
-  return check_launch("stb_icp_information");
 }
 
 }  // namespace
@@ -1188,76 +1198,89 @@ extern "C" int stb_icp_begin(void* workspace, size_t workspace_bytes,
   return check_launch("stb_icp_begin");
 }
 
-extern "C" int stb_icp_iterate(const stb_icp_level* level,
-                               const stb_icp_params* params, void* workspace,
-                               size_t workspace_bytes, int32_t first_iter,
-                               int32_t num_iters, void* stream) {
-  STB_REQUIRE(params != nullptr && workspace != nullptr, "NULL argument");
-  int rc = check_level(level, params->use_feat != 0);
+namespace {
+
+// Validates the common arguments of the fused entry points and dispatches on
+// the storage type.
+int run_fused(const stb_icp_level* level, const stb_icp_params* params,
+              void* workspace, size_t workspace_bytes, int phase, int count,
+              void* stream, const char* what) {
+  STB_REQUIRE(params != nullptr && workspace != nullptr, "%s: NULL argument",
+              what);
+  const bool jac = phase == kUpdate || phase == kLinearize || phase < 0;
+  int rc = check_level(level, jac && params->use_feat != 0);
   if (rc) return rc;
-  STB_REQUIRE(params->use_geom || params->use_feat,
-              "at least one of the geometric and feature terms must be on");
+  if (jac) {
+    STB_REQUIRE(params->use_geom || params->use_feat,
+                "%s: at least one of the geometric and feature terms must be on",
+                what);
+  }
   STB_REQUIRE(workspace_bytes >=
                   ws_bytes(level->n_src, level->height, level->width),
-              "workspace too small");
-  STB_REQUIRE(num_iters >= 0 && first_iter >= 0, "bad iteration range");
-  if (num_iters == 0) return STB_OK;
+              "%s: workspace too small", what);
   const Workspace w = carve(workspace, level->height, level->width);
   hipStream_t s = as_stream(stream);
+  if (phase < 0) {  // whole iterations
+    switch (level->storage) {
+      case STB_F32:
+        return launch_iterations<float>(*level, *params, w, count, s);
+      case STB_BF16:
+        return launch_iterations<__hip_bfloat16>(*level, *params, w, count, s);
+      default:
+        return launch_iterations<__half>(*level, *params, w, count, s);
+    }
+  }
   switch (level->storage) {
     case STB_F32:
-      return launch_iterations<float>(*level, *params, w, first_iter,
-                                      num_iters, s);
+      launch_phase<float>(*level, *params, w, Phase(phase), s);
+      break;
     case STB_BF16:
-      return launch_iterations<__hip_bfloat16>(*level, *params, w, first_iter,
-                                               num_iters, s);
+      launch_phase<__hip_bfloat16>(*level, *params, w, Phase(phase), s);
+      break;
     default:
-      return launch_iterations<__half>(*level, *params, w, first_iter,
-                                       num_iters, s);
+      launch_phase<__half>(*level, *params, w, Phase(phase), s);
   }
+  return check_launch(what);
+}
+
+}  // namespace
+
+extern "C" int stb_icp_iterate(const stb_icp_level* level,
+                               const stb_icp_params* params, void* workspace,
+                               size_t workspace_bytes, int32_t num_iters,
+                               void* stream) {
+  STB_REQUIRE(num_iters >= 0, "stb_icp_iterate: num_iters must be >= 0");
+  if (num_iters == 0) return STB_OK;
+  return run_fused(level, params, workspace, workspace_bytes, -1, num_iters,
+                   stream, "stb_icp_iterate");
+}
+
+extern "C" int stb_icp_correspond(const stb_icp_level* level,
+                                  const stb_icp_params* params,
+                                  void* workspace, size_t workspace_bytes,
+                                  void* stream) {
+  return run_fused(level, params, workspace, workspace_bytes, kCorrespond, 1,
+                   stream, "stb_icp_correspond");
+}
+
+extern "C" int stb_icp_update(const stb_icp_level* level,
+                              const stb_icp_params* params, void* workspace,
+                              size_t workspace_bytes, void* stream) {
+  return run_fused(level, params, workspace, workspace_bytes, kUpdate, 1,
+                   stream, "stb_icp_update");
 }
 
 extern "C" int stb_icp_linearize(const stb_icp_level* level,
                                  const stb_icp_params* params, void* workspace,
                                  size_t workspace_bytes, void* stream) {
-  STB_REQUIRE(params != nullptr && workspace != nullptr, "NULL argument");
-  int rc = check_level(level, params->use_feat != 0);
-  if (rc) return rc;
-  STB_REQUIRE(params->use_geom || params->use_feat,
-              "at least one of the geometric and feature terms must be on");
-  STB_REQUIRE(workspace_bytes >=
-                  ws_bytes(level->n_src, level->height, level->width),
-              "workspace too small");
-  const Workspace w = carve(workspace, level->height, level->width);
-  hipStream_t s = as_stream(stream);
-  switch (level->storage) {
-    case STB_F32:
-      return launch_linearize<float>(*level, *params, w, s);
-    case STB_BF16:
-      return launch_linearize<__hip_bfloat16>(*level, *params, w, s);
-    default:
-      return launch_linearize<__half>(*level, *params, w, s);
-  }
+  return run_fused(level, params, workspace, workspace_bytes, kLinearize, 1,
+                   stream, "stb_icp_linearize");
 }
 
 extern "C" int stb_icp_information(const stb_icp_level* level,
                                    const stb_icp_params* params,
                                    void* workspace, size_t workspace_bytes,
                                    void* stream) {
-  STB_REQUIRE(params != nullptr && workspace != nullptr, "NULL argument");
-  int rc = check_level(level, false);
-  if (rc) return rc;
-  STB_REQUIRE(workspace_bytes >=
-                  ws_bytes(level->n_src, level->height, level->width),
-              "workspace too small");
-  const Workspace w = carve(workspace, level->height, level->width);
-  hipStream_t s = as_stream(stream);
-  switch (level->storage) {
-    case STB_F32:
-      return launch_information<float>(*level, *params, w, s);
-    case STB_BF16:
-      return launch_information<__hip_bfloat16>(*level, *params, w, s);
-    default:
-      return launch_information<__half>(*level, *params, w, s);
-  }
+  return run_fused(level, params, workspace, workspace_bytes, kInformation, 1,
+                   stream, "stb_icp_information");
 }

@@ -62,7 +62,7 @@ class StbIcpState(ctypes.Structure):
         ("last_geom_r2", _f64), ("last_feat_r2", _f64),
         ("last_geom_count", _f64), ("last_feat_count", _f64),
         ("rt", _f32 * 12), ("nm", _f32 * 9), ("stopped", _i32),
-        ("iterations", _i32), ("ticket", _i32), ("pad_", _i32),
+        ("iterations", _i32), ("ticket", _i32), ("map_sel", _i32),
     ]
 
 
@@ -110,7 +110,13 @@ _SIGNATURES = {
     "stb_icp_begin": (ctypes.c_int, [_vp, _sz, _vp, _i32, _i64, _i32, _i32, _vp]),
     "stb_icp_iterate": (ctypes.c_int, [
         ctypes.POINTER(StbIcpLevel), ctypes.POINTER(StbIcpParams), _vp, _sz,
-        _i32, _i32, _vp]),
+        _i32, _vp]),
+    "stb_icp_correspond": (ctypes.c_int, [
+        ctypes.POINTER(StbIcpLevel), ctypes.POINTER(StbIcpParams), _vp, _sz,
+        _vp]),
+    "stb_icp_update": (ctypes.c_int, [
+        ctypes.POINTER(StbIcpLevel), ctypes.POINTER(StbIcpParams), _vp, _sz,
+        _vp]),
     "stb_icp_linearize": (ctypes.c_int, [
         ctypes.POINTER(StbIcpLevel), ctypes.POINTER(StbIcpParams), _vp, _sz,
         _vp]),

@@ -226,7 +226,7 @@ typedef struct stb_icp_state {
   int32_t stopped;        /* 1 after a failed Cholesky (the loop's `break`) */
   int32_t iterations;     /* accepted iterations */
   int32_t ticket;         /* block-arrival counter of the reduction kernels */
-  int32_t pad_;
+  int32_t map_sel;        /* correspondence map the next iteration uses */
 } stb_icp_state;
 
 /* Bytes of workspace needed for a level of n_src source and height*width
@@ -248,11 +248,21 @@ int stb_icp_begin(void* workspace, size_t workspace_bytes,
  * block sums the per-block partials in FP64 in a fixed order, solves the
  * 6x6 system by Cholesky in FP64 and left-multiplies exp_rt_to_matrix(x)
  * into the pose. A failed factorisation sets `stopped`, after which the
- * remaining iterations do nothing. `first_iter` numbers the iterations for
- * the map double-buffering (pass the count already enqueued since begin). */
+ * remaining iterations do nothing. Which of the two correspondence maps an
+ * iteration uses is kept in the state, so a captured graph of any number of
+ * iterations replays correctly. */
 int stb_icp_iterate(const stb_icp_level* level, const stb_icp_params* params,
-                    void* workspace, size_t workspace_bytes, int32_t first_iter,
-                    int32_t num_iters, void* stream);
+                    void* workspace, size_t workspace_bytes, int32_t num_iters,
+                    void* stream);
+
+/* The two halves of one iteration, for callers that schedule or time them
+ * separately: stb_icp_correspond builds the correspondence map at the
+ * current pose; stb_icp_update reduces the Jacobians over it, solves and
+ * updates the pose. correspond + update == one stb_icp_iterate step. */
+int stb_icp_correspond(const stb_icp_level* level, const stb_icp_params* params,
+                       void* workspace, size_t workspace_bytes, void* stream);
+int stb_icp_update(const stb_icp_level* level, const stb_icp_params* params,
+                   void* workspace, size_t workspace_bytes, void* stream);
 
 /* One linearisation at the current pose without a pose update: fills
  * last_jtj / last_jr / last_*_r2 / last_*_count with the reduced system of

@@ -1,0 +1,80 @@
+"""The C-ABI library loads without a GPU, exports every symbol the header
+declares, and the host-side helpers behave (no device compute here)."""
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from slamtb import _lib
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "slamtb_hip.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(stb_\w+)\s*\(", text)))
+
+
+def test_header_symbols_exported():
+    names = declared_functions()
+    assert len(names) >= 20
+    lib = _lib.lib()
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_binding_covers_header():
+    assert set(declared_functions()) <= set(_lib.EXPORTED)
+
+
+def test_state_struct_size_matches_library_layout():
+    # stb_icp_state: 16+16+36+36+1+1+36+6+4 doubles, 12+9 floats, 4 ints.
+    assert ctypes.sizeof(_lib.StbIcpState) == 152 * 8 + 21 * 4 + 4 * 4 + 4
+
+
+def test_workspace_size_grows_with_the_target():
+    lib = _lib.lib()
+    a = lib.stb_icp_workspace_bytes(320 * 240, 240, 320)
+    b = lib.stb_icp_workspace_bytes(640 * 480, 480, 640)
+    assert b > a >= 2 * 320 * 240 * 8
+
+
+@pytest.mark.parametrize("thr", [0.0, 0.1, math.radians(20), math.pi / 4, 1.0,
+                                 math.pi / 2, 3.0, math.pi, 3.5])
+def test_angle_gate_threshold_is_exact(thr):
+    """c* splits [-1, 1] exactly where the float acos gate flips: every float
+    within 64 ulps of c* agrees with acosf(c) >= thr (float32 libm)."""
+    lib = _lib.lib()
+    cstar = np.float32(lib.stb_angle_gate_cos_f32(thr))
+    thr32 = np.float32(thr)
+    libm = ctypes.CDLL("libm.so.6")
+    libm.acosf.restype = ctypes.c_float
+    libm.acosf.argtypes = [ctypes.c_float]
+
+    def rejects(c):
+        a = libm.acosf(float(c))
+        return not math.isnan(a) and np.float32(abs(a)) >= thr32
+
+    if cstar < -1:
+        assert not rejects(np.float32(-1.0))
+        return
+    neighbours = [cstar]
+    for direction in (np.float32(2), np.float32(-2)):
+        c = cstar
+        for _ in range(64):
+            c = np.nextafter(c, direction, dtype=np.float32)
+            neighbours.append(c)
+    for c in neighbours:
+        if -1.0 <= c <= 1.0:
+            assert rejects(c) == (c <= cstar), (c, cstar)
+
+
+def test_cpu_tensors_are_rejected():
+    from slamtb._cslamtb import Processing
+    m = torch.ones(4, 4, dtype=torch.bool)
+    with pytest.raises(RuntimeError, match="gpu tensor"):
+        Processing.erode_mask(m, torch.empty_like(m))

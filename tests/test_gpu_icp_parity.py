@@ -1,0 +1,257 @@
+"""GPU parity of the ICP hot path against the CPU oracle, through the C ABI.
+
+Bar: correspondence maps and match counts bit-exact; per-pixel partials
+equal to the oracle's FP32 values up to 2 ulp; reduced JtJ/Jr within 1e-5
+relative (norm-wise); poses within 1e-6 rad / 1e-6 m of the oracle driven
+with the same FP32 per-pixel arithmetic and FP64 sums.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from slamtb import synthetic
+from slamtb._cslamtb import CorrespondenceMap as _CMap, ICPJacobian
+from slamtb.registration import ICPOdometry
+from slamtb.registration._fused import Level, make_params, runner_for
+from slamtb.registration.correspondence_map import CorrespondenceMap
+
+from _helpers import prepared_pair, to_dev
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+ANG20 = math.radians(20)
+
+
+def small_motion(seed=11, angle=2.0, trans=0.02):
+    return synthetic.random_se3(np.random.default_rng(seed), angle, trans)
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    """320x240 pair of the single-level geometric workload."""
+    k = synthetic.config_kcam(1)
+    return prepared_pair(3, 320, 240, kcam=k)
+
+
+def gpu_map(P, T, dtype=torch.float32, dthr=0.1, athr=ANG20):
+    g = to_dev(P, DEV)
+    cm = CorrespondenceMap(dthr, athr)
+    return cm(g["src_pts"].to(dtype).view(-1, 3), g["src_nrm"].to(dtype).view(-1, 3),
+              g["src_mask"].view(-1), torch.from_numpy(T).to(DEV, dtype),
+              g["tgt_pts"].to(dtype), g["tgt_nrm"].to(dtype), g["tgt_mask"],
+              torch.from_numpy(P["kcam"]).to(DEV, dtype)).cpu().numpy()
+
+
+def cpu_map(P, T, dtype=np.float32, dthr=0.1, athr=ANG20):
+    return orc.corresp_map(P["src_pts"].astype(dtype), P["src_nrm"].astype(dtype),
+                           P["src_mask"], T, P["tgt_pts"].astype(dtype),
+                           P["tgt_nrm"].astype(dtype), P["tgt_mask"], P["kcam"],
+                           dthr, athr)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("which", ["identity", "motion"])
+def test_correspondence_bit_exact(cfg1, dtype, which):
+    T = np.eye(4) if which == "identity" else small_motion()
+    npdt = np.float32 if dtype == torch.float32 else np.float64
+    got = gpu_map(cfg1, T, dtype)
+    ref = cpu_map(cfg1, T, npdt)
+    print(f"matched pixels: {(ref != orc.CORRESP_INF).sum()}")
+    assert (ref != orc.CORRESP_INF).sum() > 1000
+    assert np.array_equal(got, ref)
+
+
+def test_correspondence_thresholds_and_ties(cfg1):
+    # A huge distance threshold and a half-turn angle threshold admit many
+    # colliding candidates, so min-merge ties and the angle edge both matter.
+    T = small_motion(5, 4.0, 0.05)
+    for dthr, athr in [(10.0, math.pi), (0.02, 0.1), (1.0, 0.0), (0.1, 3.5)]:
+        assert np.array_equal(gpu_map(cfg1, T, dthr=dthr, athr=athr),
+                              cpu_map(cfg1, T, dthr=dthr, athr=athr)), (dthr, athr)
+
+
+def test_correspondence_empty_mask(cfg1):
+    P = dict(cfg1)
+    P["src_mask"] = np.zeros_like(cfg1["src_mask"])
+    got = gpu_map(P, np.eye(4))
+    assert np.all(got == orc.CORRESP_INF)
+
+
+def test_correspondence_runs_twice_identically(cfg1):
+    T = small_motion()
+    assert np.array_equal(gpu_map(cfg1, T), gpu_map(cfg1, T))
+
+
+def test_geometric_partials(cfg1):
+    T = small_motion()
+    cmap = cpu_map(cfg1, T)
+    g = to_dev(cfg1, DEV)
+    n = cfg1["src_pts"].reshape(-1, 3).shape[0]
+    JtJ = torch.empty(n, 6, 6, device=DEV)
+    Jr = torch.empty(n, 6, device=DEV)
+    r2 = torch.empty(n, device=DEV)
+    cnt = ICPJacobian.estimate_geometric(
+        g["src_pts"].view(-1, 3), g["src_mask"].view(-1),
+        torch.from_numpy(T).float().to(DEV), g["tgt_pts"], g["tgt_nrm"],
+        torch.from_numpy(cfg1["kcam"]).to(DEV), torch.from_numpy(cmap).to(DEV),
+        JtJ, Jr, r2)
+    rj, rr, rr2, rc = orc.geometric(cfg1["src_pts"], cfg1["src_mask"],
+                                    T[:3].astype(np.float32), cfg1["tgt_pts"],
+                                    cfg1["tgt_nrm"], cfg1["kcam"], cmap)
+    assert cnt == rc and rc > 1000
+    np.testing.assert_array_max_ulp(JtJ.cpu().numpy(), rj, maxulp=2)
+    np.testing.assert_array_max_ulp(Jr.cpu().numpy(), rr, maxulp=2)
+    np.testing.assert_array_max_ulp(r2.cpu().numpy(), rr2, maxulp=2)
+
+
+def test_feature_partials():
+    P = prepared_pair(4, 160, 120, channels=3)
+    T = small_motion(2, 1.0, 0.01)
+    cmap = cpu_map(P, T, dthr=0.2, athr=math.pi / 3)
+    g = to_dev(P, DEV)
+    n = 160 * 120
+    JtJ = torch.empty(n, 6, 6, device=DEV)
+    Jr = torch.empty(n, 6, device=DEV)
+    r2 = torch.empty(n, device=DEV)
+    cnt = ICPJacobian.estimate_feature(
+        g["src_pts"].view(-1, 3), g["src_feats"].view(3, -1), g["src_mask"].view(-1),
+        torch.from_numpy(T).float().to(DEV), g["tgt_feats"],
+        torch.from_numpy(P["kcam"]).to(DEV), torch.from_numpy(cmap).to(DEV),
+        0.5, JtJ, Jr, r2)
+    rj, rr, rr2, rc = orc.feature(P["src_pts"], P["src_feats"], P["src_mask"],
+                                  T[:3].astype(np.float32), P["tgt_feats"],
+                                  P["kcam"], cmap, 0.5)
+    assert cnt == rc and rc > 100
+    np.testing.assert_array_max_ulp(JtJ.cpu().numpy(), rj, maxulp=2)
+    np.testing.assert_array_max_ulp(Jr.cpu().numpy(), rr, maxulp=2)
+    np.testing.assert_array_max_ulp(r2.cpu().numpy(), rr2, maxulp=2)
+
+
+def test_information_partials(cfg1):
+    T = small_motion()
+    cmap = cpu_map(cfg1, T)
+    g = to_dev(cfg1, DEV)
+    n = 320 * 240
+    GtG = torch.empty(n, 6, 6, device=DEV)
+    ICPJacobian.get_information_matrix(
+        g["src_pts"].view(-1, 3), g["src_mask"].view(-1),
+        torch.from_numpy(T).float().to(DEV), g["tgt_pts"],
+        torch.from_numpy(cfg1["kcam"]).to(DEV), torch.from_numpy(cmap).to(DEV), GtG)
+    ref = orc.information(cfg1["src_pts"], cfg1["src_mask"],
+                          T[:3].astype(np.float32), cfg1["tgt_pts"],
+                          cfg1["kcam"], cmap)
+    np.testing.assert_array_max_ulp(GtG.cpu().numpy(), ref, maxulp=2)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) /
+                 max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+@pytest.mark.parametrize("terms", ["geom", "feat", "both"])
+def test_fused_reduction_matches_oracle_sums(terms):
+    P = prepared_pair(9, 320, 240, channels=3)
+    T = small_motion(3, 1.0, 0.01)
+    gw = 1.0 if terms in ("geom", "both") else 0.0
+    fw = 0.5 if terms in ("feat", "both") else 0.0
+    dthr, athr, fthr = 0.1, ANG20, 2.75
+    g = to_dev(P, DEV)
+    level = Level(P["kcam"], g["src_pts"], g["src_nrm"], g["src_mask"],
+                  g["tgt_pts"], g["tgt_nrm"], g["tgt_mask"],
+                  g["src_feats"] if fw else None, g["tgt_feats"] if fw else None)
+    params = make_params(gw, fw, dthr, athr, fthr, fw > 0)
+    run = runner_for(DEV)
+    run.begin(level, torch.from_numpy(T))
+    run.linearize(level, params)
+    st = run.read_state()
+    cmap = cpu_map(P, T, dthr=dthr, athr=athr)
+    rt = T[:3].astype(np.float32)
+    JtJ = np.zeros((6, 6))
+    Jr = np.zeros(6)
+    if gw:
+        j, r, r2, c = orc.geometric(P["src_pts"], P["src_mask"], rt, P["tgt_pts"],
+                                    P["tgt_nrm"], P["kcam"], cmap)
+        JtJ += j.sum(0, dtype=np.float64) * gw * gw
+        Jr += r.sum(0, dtype=np.float64) * gw
+        assert st.last_geom_count == c
+        assert abs(st.last_geom_r2 - r2.sum(dtype=np.float64)) <= 1e-5 * r2.sum()
+    if fw:
+        j, r, r2, c = orc.feature(P["src_pts"], P["src_feats"], P["src_mask"], rt,
+                                  P["tgt_feats"], P["kcam"], cmap, fthr)
+        JtJ += j.sum(0, dtype=np.float64) * fw * fw
+        Jr += r.sum(0, dtype=np.float64) * -fw
+        assert st.last_feat_count == c
+        assert abs(st.last_feat_r2 - r2.sum(dtype=np.float64)) <= 1e-5 * r2.sum()
+    got_jtj = np.array(st.last_jtj).reshape(6, 6)
+    got_jr = np.array(st.last_jr)
+    assert rel(got_jtj, JtJ) < 1e-5, rel(got_jtj, JtJ)
+    assert rel(got_jr, Jr) < 1e-5, rel(got_jr, Jr)
+
+
+def test_fused_reduction_is_deterministic():
+    P = prepared_pair(9, 320, 240)
+    g = to_dev(P, DEV)
+    level = Level(P["kcam"], g["src_pts"], g["src_nrm"], g["src_mask"],
+                  g["tgt_pts"], g["tgt_nrm"], g["tgt_mask"])
+    params = make_params(1.0, 0.0, 0.1, ANG20, 2.75, False)
+    run = runner_for(DEV)
+    outs = []
+    for _ in range(3):
+        run.begin(level, torch.from_numpy(small_motion()))
+        run.linearize(level, params)
+        st = run.read_state()
+        outs.append((tuple(st.last_jtj), tuple(st.last_jr)))
+    assert outs[0] == outs[1] == outs[2]
+
+
+def run_icp(P, iters, gw=1.0, fw=0.0, dthr=0.1, athr=ANG20, fthr=2.75, T0=None):
+    g = to_dev(P, DEV)
+    icp = ICPOdometry(iters, geom_weight=gw, feat_weight=fw,
+                      distance_threshold=dthr, normals_angle_thresh=athr,
+                      feat_residual_thresh=fthr)
+    return icp.estimate(torch.from_numpy(P["kcam"]), g["src_pts"], g["src_nrm"],
+                        g["src_mask"], source_feats=g["src_feats"] if fw else None,
+                        target_points=g["tgt_pts"], target_mask=g["tgt_mask"],
+                        target_normals=g["tgt_nrm"],
+                        target_feats=g["tgt_feats"] if fw else None,
+                        transform=None if T0 is None else torch.from_numpy(T0))
+
+
+def test_icp_config1_matches_oracle(cfg1):
+    res = run_icp(cfg1, 10)
+    ref = orc.icp_estimate(cfg1["kcam"], cfg1["src_pts"], cfg1["src_nrm"],
+                           cfg1["src_mask"], cfg1["tgt_pts"], cfg1["tgt_nrm"],
+                           cfg1["tgt_mask"], 10, geom_weight=1, feat_weight=0,
+                           distance_threshold=0.1, normals_angle_thresh=ANG20)
+    err = orc.pose_error(res.transform.numpy(), ref["transform"])
+    print("pose error vs oracle", err, "vs ground truth",
+          orc.pose_error(res.transform.numpy(), cfg1["T_gt"]))
+    assert err[0] < 1e-6 and err[1] < 1e-6
+    assert res.iterations == ref["iters"] == 10
+    assert abs(res.match_ratio - ref["match_ratio"]) < 1e-3
+    assert abs(res.residual - ref["residual"]) <= 1e-3 * ref["residual"]
+    assert rel(res.information.double().numpy(), ref["information"]) < 1e-5
+
+
+def test_icp_features_matches_oracle():
+    P = prepared_pair(5, 320, 240, channels=3)
+    kw = dict(geom_weight=1.0, feat_weight=0.1, distance_threshold=0.1,
+              normals_angle_thresh=ANG20, feat_residual_thresh=2.75)
+    res = run_icp(P, 5, gw=1.0, fw=0.1)
+    ref = orc.icp_estimate(P["kcam"], P["src_pts"], P["src_nrm"], P["src_mask"],
+                           P["tgt_pts"], P["tgt_nrm"], P["tgt_mask"], 5,
+                           src_feats=P["src_feats"], tgt_feats=P["tgt_feats"], **kw)
+    err = orc.pose_error(res.transform.numpy(), ref["transform"])
+    print("pose error vs oracle", err)
+    assert err[0] < 1e-6 and err[1] < 1e-6
+
+
+def test_icp_zero_overlap_keeps_no_transform(cfg1):
+    far = np.eye(4)
+    far[0, 3] = 100.0  # every source point leaves the image
+    res = run_icp(cfg1, 5, T0=far)
+    assert res.transform is None and res.iterations == 0
+    assert math.isinf(res.residual)
