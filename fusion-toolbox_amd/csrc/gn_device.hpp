@@ -1,0 +1,479 @@
+// Device building blocks of the fused Gauss-Newton path, shared by the
+// single-pair kernels (gn.hip) and the batched multi-pair kernels
+// (batch.hip):
+//   * per-pixel correspondence, Jacobian and information terms over a level
+//     descriptor (any map storage type, FP32 math);
+//   * the slot reduction: 21 unique JtJ + 6 Jr + r2/count bookkeeping per
+//     lane in FP32, recursive-halving across the wavefront (32 exchanges for
+//     32 slots), FP64 across waves through LDS, and a fixed-order FP64 sum of
+//     per-block partials by the last block to arrive;
+//   * the per-pair state update: FP64 Cholesky solve and exp_rt_to_matrix.
+#pragma once
+
+#include "icp_math.hpp"
+
+namespace stb {
+
+constexpr int kSlots = 32;  // 21 JtJ + 6 Jr + r2_geom, r2_feat, n_geom, n_feat, 0
+constexpr int kWaves = kBlock / 64;
+constexpr int kGroups = kBlock / kSlots;  // slot groups of the final sum
+constexpr int kShared = (kGroups > kWaves ? kGroups : kWaves) * kSlots;
+
+// J scales and residual signs of the combined system of
+// slamtb/registration/icp.py:221-234: JtJ = fw^2 JtJ_f + gw^2 JtJ_g and
+// Jr = -fw Jr_f + gw Jr_g, i.e. J' = w J with r' = r (geometric) or -r
+// (feature).
+struct TermWeights {
+  float wg, sg;   // geometric J scale and residual scale
+  float wf, sf;   // feature J scale and residual scale
+  float fthr2;    // feature residual threshold squared
+  double gw, fw;  // weights of the residual bookkeeping
+};
+
+inline TermWeights make_weights(const stb_icp_params& P) {
+  TermWeights tw;
+  tw.wg = float(P.geom_weight);
+  tw.sg = 1.0f;
+  tw.wf = float(P.feat_weight);
+  tw.sf = -1.0f;
+  tw.fthr2 = P.feat_residual_thresh * P.feat_residual_thresh;
+  tw.gw = P.geom_weight;
+  tw.fw = P.feat_weight;
+  return tw;
+}
+
+// ---------------------------------------------------------------------------
+// Per-pixel terms
+// ---------------------------------------------------------------------------
+
+template <typename S>
+struct LevelView {
+  const S* __restrict__ sp;
+  const S* __restrict__ sn;
+  const S* __restrict__ tp;
+  const S* __restrict__ tn;
+  const S* __restrict__ sf;
+  const S* __restrict__ tf;
+  const uint8_t* __restrict__ sm;
+  const uint8_t* __restrict__ tm;
+  int64_t n, HW;
+  int H, W, C;
+  float fx, fy, cx, cy;
+};
+
+template <typename S>
+__device__ inline LevelView<S> view_of(const stb_icp_level& L) {
+  LevelView<S> v;
+  v.sp = static_cast<const S*>(L.src_points);
+  v.sn = static_cast<const S*>(L.src_normals);
+  v.tp = static_cast<const S*>(L.tgt_points);
+  v.tn = static_cast<const S*>(L.tgt_normals);
+  v.sf = static_cast<const S*>(L.src_feats);
+  v.tf = static_cast<const S*>(L.tgt_feats);
+  v.sm = L.src_mask;
+  v.tm = L.tgt_mask;
+  v.n = L.n_src;
+  v.H = L.height;
+  v.W = L.width;
+  v.HW = int64_t(L.height) * L.width;
+  v.C = L.channels;
+  v.fx = L.fx;
+  v.fy = L.fy;
+  v.cx = L.cx;
+  v.cy = L.cy;
+  return v;
+}
+
+// Correspondence of source pixel idx (src/correspondence_map.cu:38-61 with
+// RobustCorrespondence::Match): transform, project, round, bounds, target
+// mask, distance and normal-angle gates, then a 64-bit unsigned atomic min of
+// (bits(distance) << 32 | idx) into the map.
+template <typename S>
+__device__ inline void corresp_pixel(const LevelView<S>& L, const float* m,
+                                     const float* nm, int64_t idx,
+                                     unsigned long long* __restrict__ map,
+                                     float dthr2, float cstar) {
+#pragma clang fp contract(off)
+  if (!L.sm[idx]) return;
+  const V3<float> p = rigid_point(m, ld3f(L.sp, idx));
+  const V3<float> nn = rigid_normal(nm, ld3f(L.sn, idx));
+  float u, v;
+  project(L.fx, L.fy, L.cx, L.cy, p, u, v);
+  int ui, vi;
+  if (!round_in_range(u, L.W, ui) || !round_in_range(v, L.H, vi)) return;
+  const int64_t pos = int64_t(vi) * L.W + ui;
+  // Issue the three target loads together; the gates then run on registers.
+  const uint8_t valid = L.tm[pos];
+  const V3<float> q = ld3f(L.tp, pos);
+  const V3<float> qn = ld3f(L.tn, pos);
+  if (!valid) return;
+  const V3<float> d = sub3(q, p);
+  const float sq = dot3(d, d);
+  if (sq > dthr2) return;
+  if (angle_rejects(gate_cos(nn, qn), cstar)) return;
+  atomicMin(map + pos, (unsigned long long)make_key(sqrtf(sq), uint32_t(idx)));
+}
+
+// Adds one weighted term J' = w J, r' (already scaled) to the JtJ/Jr slots.
+__device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
+                                float r) {
+  float jw[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) jw[a] = j[a] * w;
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = a; b < 6; ++b) {
+      acc[k] = fmaf(jw[a], jw[b], acc[k]);
+      ++k;
+    }
+#pragma unroll
+  for (int a = 0; a < 6; ++a) acc[21 + a] = fmaf(jw[a], r, acc[21 + a]);
+}
+
+// Point-to-plane and feature terms of source pixel idx when the map names it
+// as its target pixel's owner (src/icp_jacobian_geometric.cu:54-85,
+// src/icp_jacobian_feature.cu:58-109).
+template <typename S, bool GEOM, bool FEAT>
+__device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
+                                      const unsigned long long* __restrict__ map,
+                                      int64_t idx, const TermWeights& tw,
+                                      float (&acc)[kSlots]) {
+#pragma clang fp contract(off)
+  if (!L.sm[idx]) return;
+  const V3<float> p = rigid_point(m, ld3f(L.sp, idx));
+  float u, v;
+  project(L.fx, L.fy, L.cx, L.cy, p, u, v);
+  int ui, vi;
+  if (!round_in_range(u, L.W, ui) || !round_in_range(v, L.H, vi)) return;
+  const int64_t pos = int64_t(vi) * L.W + ui;
+  if (uint32_t(map[pos]) != uint32_t(idx)) return;
+  if (GEOM) {
+    const V3<float> q = ld3f(L.tp, pos);
+    const V3<float> qn = ld3f(L.tn, pos);
+    const float r = dot3(sub3(q, p), qn);
+    const V3<float> c = cross3(p, qn);
+    const float j[6] = {qn.x, qn.y, qn.z, c.x, c.y, c.z};
+    acc_term(acc, j, tw.wg, r * tw.sg);
+    acc[27] = fmaf(r, r, acc[27]);
+    acc[29] += 1.0f;
+  }
+  if (FEAT) {
+    float j[6], rl;
+    const float sq = feature_terms<float, S, S>(L.tf, L.C, L.H, L.W, L.HW, L.sf,
+                                                L.n, idx, L.fx, L.fy, p, u, v,
+                                                j, rl);
+    if (!(sq > tw.fthr2)) {
+      acc_term(acc, j, tw.wf, rl * tw.sf);
+      acc[28] += sq;
+      acc[30] += 1.0f;
+    }
+  }
+}
+
+// G^T G of source pixel idx from its matched target point
+// (src/icp_jacobian_geometric.cu:117-158), parameter order [rot, trans].
+template <typename S>
+__device__ inline void information_pixel(const LevelView<S>& L, const float* m,
+                                         const unsigned long long* __restrict__ map,
+                                         int64_t idx, float (&acc)[kSlots]) {
+#pragma clang fp contract(off)
+  if (!L.sm[idx]) return;
+  const V3<float> p = rigid_point(m, ld3f(L.sp, idx));
+  float u, v;
+  project(L.fx, L.fy, L.cx, L.cy, p, u, v);
+  int ui, vi;
+  if (!round_in_range(u, L.W, ui) || !round_in_range(v, L.H, vi)) return;
+  const int64_t pos = int64_t(vi) * L.W + ui;
+  if (uint32_t(map[pos]) != uint32_t(idx)) return;
+  const V3<float> q = ld3f(L.tp, pos);
+  const float r0[6] = {0.f, q.z, -q.y, 1.f, 0.f, 0.f};
+  const float r1[6] = {-q.z, 0.f, q.x, 0.f, 1.f, 0.f};
+  const float r2[6] = {q.y, -q.x, 0.f, 0.f, 0.f, 1.f};
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = a; b < 6; ++b) {
+      acc[k] += r0[a] * r0[b] + r1[a] * r1[b] + r2[a] * r2[b];
+      ++k;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+
+// One recursive-halving step: lanes with bit `S` set keep the upper half of
+// their H live slots, the others the lower half, and add the partner's copy.
+template <int S, int H>
+__device__ inline void halve(float (&v)[kSlots], int lane) {
+  const bool up = (lane >> S) & 1;
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const float send = up ? v[k] : v[k + H];
+    const float keep = up ? v[k + H] : v[k];
+    v[k] = keep + __shfl_xor(send, 1 << S, 64);
+  }
+}
+
+// After this, lane l < 32 holds the wavefront total of slot bitrev5(l) in v[0]
+// (lanes 32..63 hold the same values).
+__device__ inline void wave_reduce_slots(float (&v)[kSlots], int lane) {
+  halve<0, 16>(v, lane);
+  halve<1, 8>(v, lane);
+  halve<2, 4>(v, lane);
+  halve<3, 2>(v, lane);
+  halve<4, 1>(v, lane);
+  v[0] += __shfl_xor(v[0], 32, 64);
+}
+
+__device__ inline int bitrev5(int l) {
+  return ((l & 1) << 4) | ((l & 2) << 2) | (l & 4) | ((l & 8) >> 2) |
+         ((l & 16) >> 4);
+}
+
+// Reduces the block's slots and stores them in FP64 at `out` (32 doubles),
+// made visible device-wide. `sh` is LDS of kShared doubles.
+__device__ inline void block_partial(float (&acc)[kSlots], double* out,
+                                     double* sh) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  wave_reduce_slots(acc, lane);
+  if (lane < 32) sh[wave * kSlots + bitrev5(lane)] = double(acc[0]);
+  __syncthreads();
+  if (threadIdx.x < kSlots) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s += sh[w * kSlots + threadIdx.x];
+    out[threadIdx.x] = s;
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+// Arrival ticket: true in every thread of the block that brings `counter` to
+// `expected` (the last of `expected` + 1 arrivals).
+__device__ inline bool arrive_last(int32_t* counter, int expected,
+                                   int* s_flag) {
+  if (threadIdx.x == 0) *s_flag = (atomicAdd(counter, 1) == expected);
+  __syncthreads();
+  const bool last = *s_flag != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+// Fixed-order FP64 sum of `count` partials (32 doubles each) into tot[32]:
+// group g of kGroups adds partials g, g + kGroups, ..., then the groups are
+// added in order -- the same order whichever block performs it.
+__device__ inline void sum_partials(const double* partials, int count,
+                                    double* sh, double* tot) {
+  const int slot = threadIdx.x & (kSlots - 1);
+  const int grp = threadIdx.x / kSlots;
+  double s = 0.0;
+  for (int b = grp; b < count; b += kGroups)
+    s += partials[int64_t(b) * kSlots + slot];
+  __syncthreads();
+  sh[grp * kSlots + slot] = s;
+  __syncthreads();
+  if (threadIdx.x < kSlots) {
+    double t = 0.0;
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) t += sh[g * kSlots + threadIdx.x];
+    tot[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Pose state
+// ---------------------------------------------------------------------------
+
+__device__ inline void set_pose_floats(stb_icp_state* st) {
+  Rigid<float> r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r.m[i * 4 + j] = float(st->transform[i * 4 + j]);
+  make_normal_matrix(r);
+#pragma unroll
+  for (int i = 0; i < 12; ++i) st->rt[i] = r.m[i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) st->nm[i] = r.nm[i];
+}
+
+// exp_rt_to_matrix in FP64 (src/exp_rt.hpp:27-65, src/exp_rt_to_matrix.cu:22-36):
+// Rodrigues rotation of w = x[3:6], translation x[0:3] used as is.
+__device__ inline void exp_rt_f64(const double x[6], double U[12]) {
+  const double wx = x[3], wy = x[4], wz = x[5];
+  const double sq = wx * wx + (wy * wy + wz * wz);
+  const double th = sqrt(sq);
+  double ax = wx, ay = wy, az = wz;
+  if (sq > 0.0) {
+    ax = wx / th;
+    ay = wy / th;
+    az = wz / th;
+  }
+  const double s = sin(th), c = cos(th), k = 1.0 - c;
+  U[0] = k * ax * ax + c;
+  U[1] = k * ax * ay - s * az;
+  U[2] = k * ax * az + s * ay;
+  U[4] = k * ax * ay + s * az;
+  U[5] = k * ay * ay + c;
+  U[6] = k * ay * az - s * ax;
+  U[8] = k * ax * az - s * ay;
+  U[9] = k * ay * az + s * ax;
+  U[10] = k * az * az + c;
+  U[3] = x[0];
+  U[7] = x[1];
+  U[11] = x[2];
+}
+
+// Stores the reduced system of one linearisation in the state.
+__device__ inline void store_linearization(stb_icp_state* st, const double* tot) {
+  int k = 0;
+#pragma unroll 1
+  for (int a = 0; a < 6; ++a)
+#pragma unroll 1
+    for (int c = a; c < 6; ++c) {
+      st->last_jtj[a * 6 + c] = tot[k];
+      st->last_jtj[c * 6 + a] = tot[k];
+      ++k;
+    }
+#pragma unroll 1
+  for (int a = 0; a < 6; ++a) st->last_jr[a] = tot[21 + a];
+  st->last_geom_r2 = tot[27];
+  st->last_feat_r2 = tot[28];
+  st->last_geom_count = tot[29];
+  st->last_feat_count = tot[30];
+}
+
+// One Gauss-Newton update from the reduced totals (slamtb/registration/icp.py:
+// 236-256): FP64 Cholesky of JtJ (a non-positive pivot is the reference's
+// `except: break`), solve JtJ x = Jr, T <- exp_rt(x) T. Runs in one thread
+// on matrices kept in LDS (`w` holds 36 + 36 + 6 + 12 doubles) with its
+// loops left rolled, so that the kernel's register budget is set by the
+// per-pixel loop rather than by this tail. Flips map_sel on success.
+__device__ inline void gn_finalize(stb_icp_state* st, const double* tot,
+                                   double gw, double fw, double* w) {
+  double* A = w;
+  double* L = w + 36;
+  double* x = w + 72;
+  double* U = w + 78;
+  store_linearization(st, tot);
+#pragma unroll 1
+  for (int i = 0; i < 36; ++i) {
+    A[i] = st->last_jtj[i];
+    L[i] = 0.0;
+  }
+#pragma unroll 1
+  for (int j = 0; j < 6; ++j) {
+    double s = A[j * 6 + j];
+#pragma unroll 1
+    for (int q = 0; q < j; ++q) s -= L[j * 6 + q] * L[j * 6 + q];
+    if (!(s > 0.0)) {
+      st->stopped = 1;
+      return;
+    }
+    const double d = sqrt(s);
+    L[j * 6 + j] = d;
+#pragma unroll 1
+    for (int i = j + 1; i < 6; ++i) {
+      double t = A[i * 6 + j];
+#pragma unroll 1
+      for (int q = 0; q < j; ++q) t -= L[i * 6 + q] * L[j * 6 + q];
+      L[i * 6 + j] = t / d;
+    }
+  }
+#pragma unroll 1
+  for (int i = 0; i < 6; ++i) {  // L y = b (y kept in x)
+    double t = tot[21 + i];
+#pragma unroll 1
+    for (int q = 0; q < i; ++q) t -= L[i * 6 + q] * x[q];
+    x[i] = t / L[i * 6 + i];
+  }
+#pragma unroll 1
+  for (int i = 5; i >= 0; --i) {  // L^T x = y
+    double t = x[i];
+#pragma unroll 1
+    for (int q = i + 1; q < 6; ++q) t -= L[q * 6 + i] * x[q];
+    x[i] = t / L[i * 6 + i];
+  }
+  exp_rt_f64(x, U);
+  // T <- U T (rows 0..2; the last row stays [0 0 0 1]); L is free again and
+  // stages the new rows.
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i)
+#pragma unroll 1
+    for (int j = 0; j < 4; ++j) {
+      double t = U[i * 4 + 0] * st->transform[0 * 4 + j] +
+                 U[i * 4 + 1] * st->transform[1 * 4 + j] +
+                 U[i * 4 + 2] * st->transform[2 * 4 + j];
+      if (j == 3) t += U[i * 4 + 3];
+      L[i * 4 + j] = t;
+    }
+#pragma unroll 1
+  for (int i = 0; i < 12; ++i) {
+    st->transform[i] = L[i];
+    st->best_transform[i] = L[i];
+  }
+#pragma unroll 1
+  for (int i = 0; i < 36; ++i) st->best_jtj[i] = A[i];
+  const double count = fmax(tot[29], tot[30]);
+  st->best_residual = (fw * tot[28] + gw * tot[27]) / count;
+  st->best_match_count = count;
+  st->iterations += 1;
+  st->map_sel ^= 1;
+  set_pose_floats(st);
+}
+
+__device__ inline void store_information(stb_icp_state* st, const double* tot) {
+  int k = 0;
+#pragma unroll 1
+  for (int a = 0; a < 6; ++a)
+#pragma unroll 1
+    for (int c = a; c < 6; ++c) {
+      st->information[a * 6 + c] = tot[k];
+      st->information[c * 6 + a] = tot[k];
+      ++k;
+    }
+}
+
+// Resets one state to pose T0 (row-major 4x4, NULL = identity) or, with
+// `chain`, to its own last accepted pose (identity if none).
+__device__ inline void reset_state(stb_icp_state* st,
+                                   const double* __restrict__ T0, int chain) {
+  double T[16];
+  const bool from_prev = chain && st->iterations > 0;
+  for (int i = 0; i < 16; ++i) {
+    const double eye = (i % 5 == 0) ? 1.0 : 0.0;
+    if (chain)
+      T[i] = from_prev ? st->best_transform[i] : eye;
+    else
+      T[i] = T0 ? T0[i] : eye;
+  }
+  for (int i = 0; i < 16; ++i) {
+    st->transform[i] = T[i];
+    st->best_transform[i] = T[i];
+  }
+  for (int i = 0; i < 36; ++i) {
+    st->best_jtj[i] = 0.0;
+    st->last_jtj[i] = 0.0;
+    st->information[i] = 0.0;
+  }
+  for (int i = 0; i < 6; ++i) st->last_jr[i] = 0.0;
+  st->best_residual = __builtin_inf();
+  st->best_match_count = 0.0;
+  st->last_geom_r2 = st->last_feat_r2 = 0.0;
+  st->last_geom_count = st->last_feat_count = 0.0;
+  st->stopped = 0;
+  st->iterations = 0;
+  st->ticket = 0;
+  st->map_sel = 0;
+  set_pose_floats(st);
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace stb

@@ -49,7 +49,8 @@ class StbIcpParams(ctypes.Structure):
         ("geom_weight", _f64), ("feat_weight", _f64),
         ("distance_thresh", _f64), ("normal_angle_thresh", _f64),
         ("feat_residual_thresh", _f32), ("use_geom", _i32),
-        ("use_feat", _i32), ("feat_mode", _i32),
+        ("use_feat", _i32), ("feat_mode", _i32), ("grid_update", _i32),
+        ("grid_corresp", _i32),
     ]
 
 

@@ -84,7 +84,7 @@ def make_params(geom_weight, feat_weight, distance_threshold,
                              float(distance_threshold),
                              float(normals_angle_thresh),
                              float(feat_residual_thresh), int(use_geom),
-                             int(bool(use_feat)), 0)
+                             int(bool(use_feat)), 0, 0, 0)
 
 
 class GNRunner:

@@ -207,6 +207,8 @@ typedef struct stb_icp_params {
   int32_t use_geom;     /* geom_weight > 0 */
   int32_t use_feat;     /* feat_weight > 0 and both feature maps given */
   int32_t feat_mode;    /* 0: last channel's J (reference), 1: all channels */
+  int32_t grid_update;  /* workgroups of the Jacobian kernel (0 = default) */
+  int32_t grid_corresp; /* workgroups of the correspondence kernel (0 = default) */
 } stb_icp_params;
 
 /* Device-resident result, at offset 0 of the workspace. Fields mirror
