@@ -1,0 +1,481 @@
+// Batched Gauss-Newton over many frame pairs of uneven resolution.
+//
+// A batch plan (built on the host by stb_icp_batch_plan, uploaded by the
+// caller) lists one level descriptor per pair and cuts every pair's source
+// pixels into equal tiles. Each iteration is two persistent launches over the
+// flattened tile list of all pairs -- correspondence, then Jacobians and
+// reduction -- whatever the pair count or resolutions. A workgroup walks
+// tiles t = blockIdx.x, blockIdx.x + gridDim.x, ...; for every Jacobian tile
+// it writes one FP64 partial and takes an arrival ticket on its pair; the
+// workgroup that completes a pair sums that pair's partials in tile order,
+// solves the pair's 6x6 system in FP64 and updates that pair's pose. All
+// pairs therefore advance one iteration per launch pair, with no host
+// involvement, and a whole multi-level loop can be captured in one hipGraph.
+#include "gn_device.hpp"
+
+#include <cstring>
+#include <vector>
+
+namespace stb {
+namespace {
+
+struct PlanHeader {
+  int32_t num_pairs;
+  int32_t total_tiles;
+  int32_t tile_px;
+  int32_t storage;
+  int64_t total_map;
+  int64_t reserved;
+};
+
+// Plan image: header | levels[P] | map_off[P] (int64) | tile_start[P + 1]
+// (int32) | tile_pair[total_tiles] (int32), each part 64-byte aligned.
+struct PlanLayout {
+  size_t levels, map_off, tile_start, tile_pair, bytes;
+};
+
+__host__ __device__ inline size_t align64(size_t x) {
+  return (x + 63) & ~size_t(63);
+}
+
+__host__ __device__ inline PlanLayout plan_layout(int P, int tiles) {
+  PlanLayout l;
+  l.levels = align64(sizeof(PlanHeader));
+  l.map_off = l.levels + align64(sizeof(stb_icp_level) * size_t(P));
+  l.tile_start = l.map_off + align64(sizeof(int64_t) * size_t(P));
+  l.tile_pair = l.tile_start + align64(sizeof(int32_t) * size_t(P + 1));
+  l.bytes = l.tile_pair + align64(sizeof(int32_t) * size_t(tiles));
+  return l;
+}
+
+struct PlanView {
+  int P, tiles, tile_px;
+  const stb_icp_level* levels;
+  const int64_t* map_off;
+  const int32_t* tile_start;
+  const int32_t* tile_pair;
+};
+
+__device__ inline PlanView plan_view(const void* plan) {
+  const char* b = static_cast<const char*>(plan);
+  const PlanHeader* h = reinterpret_cast<const PlanHeader*>(b);
+  const PlanLayout l = plan_layout(h->num_pairs, h->total_tiles);
+  PlanView v;
+  v.P = h->num_pairs;
+  v.tiles = h->total_tiles;
+  v.tile_px = h->tile_px;
+  v.levels = reinterpret_cast<const stb_icp_level*>(b + l.levels);
+  v.map_off = reinterpret_cast<const int64_t*>(b + l.map_off);
+  v.tile_start = reinterpret_cast<const int32_t*>(b + l.tile_start);
+  v.tile_pair = reinterpret_cast<const int32_t*>(b + l.tile_pair);
+  return v;
+}
+
+// Scratch: map0[total_map] | map1[total_map] | partials[tiles][32] |
+// tickets[P].
+struct Scratch {
+  unsigned long long* map0;
+  unsigned long long* map1;
+  double* partials;
+  int32_t* tickets;
+  size_t bytes;
+};
+
+Scratch carve_scratch(void* base, int P, int tiles, int64_t total_map) {
+  Scratch s;
+  char* p = static_cast<char*>(base);
+  s.map0 = reinterpret_cast<unsigned long long*>(p);
+  p += align256(size_t(total_map) * 8);
+  s.map1 = reinterpret_cast<unsigned long long*>(p);
+  p += align256(size_t(total_map) * 8);
+  s.partials = reinterpret_cast<double*>(p);
+  p += align256(size_t(tiles) * kSlots * sizeof(double));
+  s.tickets = reinterpret_cast<int32_t*>(p);
+  p += align256(size_t(P) * sizeof(int32_t));
+  s.bytes = size_t(p - static_cast<char*>(base));
+  return s;
+}
+
+__device__ inline unsigned long long* pair_map(const stb_icp_state* st,
+                                               unsigned long long* maps0,
+                                               unsigned long long* maps1,
+                                               int64_t off, int spare) {
+  return ((st->map_sel ^ spare) ? maps1 : maps0) + off;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(kBlock)
+    batch_corresp_kernel(const void* __restrict__ plan,
+                         const stb_icp_state* __restrict__ states,
+                         unsigned long long* __restrict__ maps0,
+                         unsigned long long* __restrict__ maps1, float dthr2,
+                         float cstar, int fixed_map) {
+  const PlanView pv = plan_view(plan);
+  for (int t = blockIdx.x; t < pv.tiles; t += gridDim.x) {
+    const int p = pv.tile_pair[t];
+    const stb_icp_state* st = states + p;
+    if (!fixed_map && st->stopped) continue;
+    const LevelView<S> L = view_of<S>(pv.levels[p]);
+    float m[12], nm[9];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) nm[i] = st->nm[i];
+    unsigned long long* map =
+        pair_map(st, maps0, maps1, pv.map_off[p], fixed_map);
+    const int64_t base = int64_t(t - pv.tile_start[p]) * pv.tile_px;
+    for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
+      const int64_t idx = base + k;
+      if (idx < L.n) corresp_pixel(L, m, nm, idx, map, dthr2, cstar);
+    }
+  }
+}
+
+template <typename S, bool GEOM, bool FEAT>
+__global__ void __launch_bounds__(kBlock)
+    batch_jacobian_kernel(const void* __restrict__ plan,
+                          stb_icp_state* __restrict__ states,
+                          unsigned long long* __restrict__ maps0,
+                          unsigned long long* __restrict__ maps1,
+                          double* __restrict__ partials,
+                          int32_t* __restrict__ tickets, TermWeights tw,
+                          int do_solve, int fixed_map) {
+  __shared__ double sh[kShared];
+  __shared__ double tot[kSlots];
+  __shared__ int s_last;
+  const PlanView pv = plan_view(plan);
+  for (int t = blockIdx.x; t < pv.tiles; t += gridDim.x) {
+    const int p = pv.tile_pair[t];
+    stb_icp_state* st = states + p;
+    if (!fixed_map && st->stopped) continue;
+    const LevelView<S> L = view_of<S>(pv.levels[p]);
+    const int64_t off = pv.map_off[p];
+    const unsigned long long* map = pair_map(st, maps0, maps1, off, fixed_map);
+    const int first = pv.tile_start[p];
+    const int ntiles = pv.tile_start[p + 1] - first;
+    const int64_t base = int64_t(t - first) * pv.tile_px;
+    if (!fixed_map) {
+      // Clear this tile's share of the pair's other map for the next
+      // iteration; the pair's last tile also clears target pixels beyond
+      // the source count.
+      unsigned long long* next = pair_map(st, maps0, maps1, off, 1);
+      int64_t end = base + pv.tile_px;
+      if (t - first == ntiles - 1 && end < L.HW) end = L.HW;
+      if (end > L.HW) end = L.HW;
+      for (int64_t i = base + threadIdx.x; i < end; i += blockDim.x)
+        next[i] = kKeySentinel;
+    }
+    float m[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
+    float acc[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
+    for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
+      const int64_t idx = base + k;
+      if (idx < L.n) jacobian_pixel<S, GEOM, FEAT>(L, m, map, idx, tw, acc);
+    }
+    block_partial(acc, partials + int64_t(t) * kSlots, sh);
+    if (arrive_last(&tickets[p], ntiles - 1, &s_last)) {
+      sum_partials(partials + int64_t(first) * kSlots, ntiles, sh, tot);
+      if (threadIdx.x == 0) {
+        tickets[p] = 0;
+        if (do_solve)
+          gn_finalize(st, tot, tw.gw, tw.fw, sh);
+        else
+          store_linearization(st, tot);
+      }
+    }
+    __syncthreads();  // LDS is reused by the next tile
+  }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(kBlock)
+    batch_information_kernel(const void* __restrict__ plan,
+                             stb_icp_state* __restrict__ states,
+                             unsigned long long* __restrict__ maps0,
+                             unsigned long long* __restrict__ maps1,
+                             double* __restrict__ partials,
+                             int32_t* __restrict__ tickets) {
+  __shared__ double sh[kShared];
+  __shared__ double tot[kSlots];
+  __shared__ int s_last;
+  const PlanView pv = plan_view(plan);
+  for (int t = blockIdx.x; t < pv.tiles; t += gridDim.x) {
+    const int p = pv.tile_pair[t];
+    stb_icp_state* st = states + p;
+    const LevelView<S> L = view_of<S>(pv.levels[p]);
+    const unsigned long long* map =
+        pair_map(st, maps0, maps1, pv.map_off[p], 1);
+    const int first = pv.tile_start[p];
+    const int ntiles = pv.tile_start[p + 1] - first;
+    const int64_t base = int64_t(t - first) * pv.tile_px;
+    float m[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
+    float acc[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
+    for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
+      const int64_t idx = base + k;
+      if (idx < L.n) information_pixel(L, m, map, idx, acc);
+    }
+    block_partial(acc, partials + int64_t(t) * kSlots, sh);
+    if (arrive_last(&tickets[p], ntiles - 1, &s_last)) {
+      sum_partials(partials + int64_t(first) * kSlots, ntiles, sh, tot);
+      if (threadIdx.x == 0) {
+        tickets[p] = 0;
+        store_information(st, tot);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void batch_begin_kernel(const void* __restrict__ plan,
+                                   stb_icp_state* __restrict__ states,
+                                   const double* __restrict__ T0, int chain,
+                                   int32_t* __restrict__ tickets) {
+  const PlanView pv = plan_view(plan);
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pv.P) return;
+  reset_state(states + p, T0 ? T0 + 16 * p : nullptr, chain);
+  tickets[p] = 0;
+}
+
+// Clears map entries [0, total) of both maps (spare == 0) or, per pair
+// (blockIdx.y), the pair's spare map (spare == 1).
+__global__ void __launch_bounds__(kBlock)
+    batch_fill_kernel(const void* __restrict__ plan,
+                      const stb_icp_state* __restrict__ states,
+                      unsigned long long* __restrict__ maps0,
+                      unsigned long long* __restrict__ maps1, int64_t total,
+                      int spare) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int64_t start = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (!spare) {
+    for (int64_t i = start; i < total; i += stride) {
+      maps0[i] = kKeySentinel;
+      maps1[i] = kKeySentinel;
+    }
+    return;
+  }
+  const PlanView pv = plan_view(plan);
+  const int p = blockIdx.y;
+  if (p >= pv.P) return;
+  const stb_icp_level& L = pv.levels[p];
+  const int64_t hw = int64_t(L.height) * L.width;
+  unsigned long long* m = pair_map(states + p, maps0, maps1, pv.map_off[p], 1);
+  for (int64_t i = start; i < hw; i += stride) m[i] = kKeySentinel;
+}
+
+int check_sizes(const stb_icp_batch_sizes* z) {
+  STB_REQUIRE(z != nullptr, "batch sizes are NULL");
+  STB_REQUIRE(z->num_pairs > 0 && z->total_tiles > 0 && z->tile_px > 0,
+              "empty batch plan");
+  STB_REQUIRE(z->storage == STB_F32 || z->storage == STB_BF16 ||
+                  z->storage == STB_F16,
+              "bad batch storage");
+  return STB_OK;
+}
+
+int grid_tiles(const stb_icp_params* P, int tiles, int dflt) {
+  const int cap = (P && P->grid_update > 0) ? P->grid_update : dflt;
+  return tiles < cap ? tiles : cap;
+}
+
+template <typename S, bool GEOM, bool FEAT>
+void launch_batch_jacobian(const void* plan, stb_icp_state* states,
+                           const Scratch& sc, int grid, const TermWeights& tw,
+                           int do_solve, int fixed, hipStream_t s) {
+  hipLaunchKernelGGL((batch_jacobian_kernel<S, GEOM, FEAT>), dim3(grid),
+                     dim3(kBlock), 0, s, plan, states, sc.map0, sc.map1,
+                     sc.partials, sc.tickets, tw, do_solve, fixed);
+}
+
+enum BatchPhase { kIterate, kLinearize, kInformation };
+
+template <typename S>
+void launch_batch(const void* plan, const stb_icp_batch_sizes& z,
+                  const stb_icp_params& P, stb_icp_state* states,
+                  const Scratch& sc, BatchPhase phase, int count,
+                  hipStream_t s) {
+  const float dthr2 = float(P.distance_thresh * P.distance_thresh);
+  const float cstar = angle_gate_cos_f32(P.normal_angle_thresh);
+  const int gc = (P.grid_corresp > 0) ? (z.total_tiles < P.grid_corresp
+                                             ? z.total_tiles
+                                             : P.grid_corresp)
+                                      : (z.total_tiles < 8192 ? z.total_tiles : 8192);
+  const int gj = grid_tiles(&P, z.total_tiles, 1024);
+  const TermWeights tw = make_weights(P);
+  const int fixed = phase == kIterate ? 0 : 1;
+  const int reps = phase == kIterate ? count : 1;
+  for (int it = 0; it < reps; ++it) {
+    if (fixed) {
+      dim3 g(int(grid_for(1 << 20, kBlock, 256)), z.num_pairs);
+      hipLaunchKernelGGL(batch_fill_kernel, g, dim3(kBlock), 0, s, plan,
+                         states, sc.map0, sc.map1, z.total_map, 1);
+    }
+    hipLaunchKernelGGL(batch_corresp_kernel<S>, dim3(gc), dim3(kBlock), 0, s,
+                       plan, states, sc.map0, sc.map1, dthr2, cstar, fixed);
+    if (phase == kInformation) {
+      hipLaunchKernelGGL(batch_information_kernel<S>, dim3(gj), dim3(kBlock), 0,
+                         s, plan, states, sc.map0, sc.map1, sc.partials,
+                         sc.tickets);
+      continue;
+    }
+    const int solve = phase == kIterate ? 1 : 0;
+    if (P.use_geom && P.use_feat)
+      launch_batch_jacobian<S, true, true>(plan, states, sc, gj, tw, solve,
+                                           fixed, s);
+    else if (P.use_geom)
+      launch_batch_jacobian<S, true, false>(plan, states, sc, gj, tw, solve,
+                                            fixed, s);
+    else
+      launch_batch_jacobian<S, false, true>(plan, states, sc, gj, tw, solve,
+                                            fixed, s);
+  }
+}
+
+int run_batch(const void* plan, const stb_icp_batch_sizes* z,
+              const stb_icp_params* params, void* states, void* scratch,
+              size_t scratch_bytes, BatchPhase phase, int count, void* stream,
+              const char* what) {
+  int rc = check_sizes(z);
+  if (rc) return rc;
+  STB_REQUIRE(plan && params && states && scratch, "%s: NULL argument", what);
+  if (phase != kInformation) {
+    STB_REQUIRE(params->use_geom || params->use_feat,
+                "%s: at least one of the geometric and feature terms must be on",
+                what);
+  }
+  const Scratch sc =
+      carve_scratch(scratch, z->num_pairs, z->total_tiles, z->total_map);
+  STB_REQUIRE(scratch_bytes >= sc.bytes, "%s: scratch too small (%zu < %zu)",
+              what, scratch_bytes, sc.bytes);
+  stb_icp_state* st = static_cast<stb_icp_state*>(states);
+  hipStream_t s = as_stream(stream);
+  switch (z->storage) {
+    case STB_F32:
+      launch_batch<float>(plan, *z, *params, st, sc, phase, count, s);
+      break;
+    case STB_BF16:
+      launch_batch<__hip_bfloat16>(plan, *z, *params, st, sc, phase, count, s);
+      break;
+    default:
+      launch_batch<__half>(plan, *z, *params, st, sc, phase, count, s);
+  }
+  return check_launch(what);
+}
+
+}  // namespace
+}  // namespace stb
+
+using namespace stb;
+
+extern "C" int stb_icp_batch_plan(int32_t num_pairs,
+                                  const stb_icp_level* levels, int32_t tile_px,
+                                  void* plan_host, size_t plan_capacity,
+                                  stb_icp_batch_sizes* sizes) {
+  STB_REQUIRE(num_pairs > 0 && levels != nullptr && sizes != nullptr,
+              "stb_icp_batch_plan: bad arguments");
+  STB_REQUIRE(tile_px > 0 && tile_px % kBlock == 0,
+              "stb_icp_batch_plan: tile_px must be a positive multiple of %d",
+              kBlock);
+  const int32_t storage = levels[0].storage;
+  std::vector<int32_t> start(size_t(num_pairs) + 1, 0);
+  std::vector<int64_t> off(size_t(num_pairs), 0);
+  int64_t tiles = 0, maps = 0;
+  for (int p = 0; p < num_pairs; ++p) {
+    const stb_icp_level& L = levels[p];
+    STB_REQUIRE(L.src_points && L.src_normals && L.src_mask && L.tgt_points &&
+                    L.tgt_normals && L.tgt_mask,
+                "stb_icp_batch_plan: pair %d has a NULL array", p);
+    STB_REQUIRE(L.n_src > 0 && L.height > 0 && L.width > 0,
+                "stb_icp_batch_plan: pair %d is empty", p);
+    STB_REQUIRE(L.storage == storage,
+                "stb_icp_batch_plan: all pairs must share one storage type");
+    start[p] = int32_t(tiles);
+    off[p] = maps;
+    tiles += (L.n_src + tile_px - 1) / tile_px;
+    maps += int64_t(L.height) * L.width;
+    STB_REQUIRE(tiles < (int64_t(1) << 30), "stb_icp_batch_plan: too many tiles");
+  }
+  start[num_pairs] = int32_t(tiles);
+  const PlanLayout l = plan_layout(num_pairs, int(tiles));
+  const Scratch sc = carve_scratch(nullptr, num_pairs, int(tiles), maps);
+  sizes->num_pairs = num_pairs;
+  sizes->total_tiles = int32_t(tiles);
+  sizes->tile_px = tile_px;
+  sizes->storage = storage;
+  sizes->total_map = maps;
+  sizes->plan_bytes = l.bytes;
+  sizes->scratch_bytes = sc.bytes;
+  if (plan_host == nullptr) return STB_OK;  // size query
+  STB_REQUIRE(plan_capacity >= l.bytes, "stb_icp_batch_plan: buffer too small");
+  char* b = static_cast<char*>(plan_host);
+  std::memset(b, 0, l.bytes);
+  PlanHeader h{num_pairs, int32_t(tiles), tile_px, storage, maps, 0};
+  std::memcpy(b, &h, sizeof(h));
+  std::memcpy(b + l.levels, levels, sizeof(stb_icp_level) * size_t(num_pairs));
+  std::memcpy(b + l.map_off, off.data(), sizeof(int64_t) * size_t(num_pairs));
+  std::memcpy(b + l.tile_start, start.data(),
+              sizeof(int32_t) * size_t(num_pairs + 1));
+  int32_t* tp = reinterpret_cast<int32_t*>(b + l.tile_pair);
+  for (int p = 0; p < num_pairs; ++p)
+    for (int32_t t = start[p]; t < start[p + 1]; ++t) tp[t] = p;
+  return STB_OK;
+}
+
+extern "C" int stb_icp_batch_begin(const void* plan,
+                                   const stb_icp_batch_sizes* sizes,
+                                   void* states, const double* transforms0,
+                                   int32_t chain, void* scratch,
+                                   size_t scratch_bytes, void* stream) {
+  int rc = check_sizes(sizes);
+  if (rc) return rc;
+  STB_REQUIRE(plan && states && scratch, "stb_icp_batch_begin: NULL argument");
+  const Scratch sc = carve_scratch(scratch, sizes->num_pairs,
+                                   sizes->total_tiles, sizes->total_map);
+  STB_REQUIRE(scratch_bytes >= sc.bytes, "stb_icp_batch_begin: scratch too small");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(batch_begin_kernel, dim3((sizes->num_pairs + 63) / 64),
+                     dim3(64), 0, s, plan, static_cast<stb_icp_state*>(states),
+                     transforms0, chain, sc.tickets);
+  hipLaunchKernelGGL(batch_fill_kernel,
+                     dim3(grid_for(sizes->total_map, kBlock, 4096)),
+                     dim3(kBlock), 0, s, plan,
+                     static_cast<const stb_icp_state*>(states), sc.map0,
+                     sc.map1, sizes->total_map, 0);
+  return check_launch("stb_icp_batch_begin");
+}
+
+extern "C" int stb_icp_batch_iterate(const void* plan,
+                                     const stb_icp_batch_sizes* sizes,
+                                     const stb_icp_params* params, void* states,
+                                     void* scratch, size_t scratch_bytes,
+                                     int32_t num_iters, void* stream) {
+  STB_REQUIRE(num_iters >= 0, "stb_icp_batch_iterate: num_iters must be >= 0");
+  if (num_iters == 0) return STB_OK;
+  return run_batch(plan, sizes, params, states, scratch, scratch_bytes, kIterate,
+                   num_iters, stream, "stb_icp_batch_iterate");
+}
+
+extern "C" int stb_icp_batch_linearize(const void* plan,
+                                       const stb_icp_batch_sizes* sizes,
+                                       const stb_icp_params* params,
+                                       void* states, void* scratch,
+                                       size_t scratch_bytes, void* stream) {
+  return run_batch(plan, sizes, params, states, scratch, scratch_bytes,
+                   kLinearize, 1, stream, "stb_icp_batch_linearize");
+}
+
+extern "C" int stb_icp_batch_information(const void* plan,
+                                         const stb_icp_batch_sizes* sizes,
+                                         const stb_icp_params* params,
+                                         void* states, void* scratch,
+                                         size_t scratch_bytes, void* stream) {
+  return run_batch(plan, sizes, params, states, scratch, scratch_bytes,
+                   kInformation, 1, stream, "stb_icp_batch_information");
+}

@@ -67,6 +67,17 @@ class StbIcpState(ctypes.Structure):
     ]
 
 
+class StbIcpBatchSizes(ctypes.Structure):
+    _fields_ = [
+        ("num_pairs", _i32), ("total_tiles", _i32), ("tile_px", _i32),
+        ("storage", _i32), ("total_map", _i64), ("plan_bytes", _sz),
+        ("scratch_bytes", _sz),
+    ]
+
+
+_BATCH = ctypes.POINTER(StbIcpBatchSizes)
+_PARAMS = ctypes.POINTER(StbIcpParams)
+
 _SIGNATURES = {
     "stb_last_error": (ctypes.c_char_p, []),
     "stb_abi_version": (ctypes.c_int, []),
@@ -124,6 +135,16 @@ _SIGNATURES = {
     "stb_icp_information": (ctypes.c_int, [
         ctypes.POINTER(StbIcpLevel), ctypes.POINTER(StbIcpParams), _vp, _sz,
         _vp]),
+    "stb_icp_batch_plan": (ctypes.c_int, [
+        _i32, ctypes.POINTER(StbIcpLevel), _i32, _vp, _sz, _BATCH]),
+    "stb_icp_batch_begin": (ctypes.c_int, [
+        _vp, _BATCH, _vp, _vp, _i32, _vp, _sz, _vp]),
+    "stb_icp_batch_iterate": (ctypes.c_int, [
+        _vp, _BATCH, _PARAMS, _vp, _vp, _sz, _i32, _vp]),
+    "stb_icp_batch_linearize": (ctypes.c_int, [
+        _vp, _BATCH, _PARAMS, _vp, _vp, _sz, _vp]),
+    "stb_icp_batch_information": (ctypes.c_int, [
+        _vp, _BATCH, _PARAMS, _vp, _vp, _sz, _vp]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

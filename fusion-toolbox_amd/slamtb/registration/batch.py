@@ -1,0 +1,115 @@
+"""Batched multiscale ICP: many frame pairs of uneven resolution registered
+together on one GPU (the batch entry points of libslamtb_hip.so).
+
+For every pyramid level, all pairs' source pixels form one flattened tile
+list; each Gauss-Newton iteration is one correspondence launch and one
+Jacobian/reduction launch over that list, and the workgroup that completes
+a pair solves that pair's system and updates its pose on the device. The
+result equals running ``MultiscaleRegistration.estimate_frame`` on each
+pair with the same estimators (same per-pixel arithmetic; only the order of
+the FP64 partial sums differs).
+"""
+import ctypes
+
+import torch
+
+from slamtb import _lib
+
+from ._fused import Level, STATE_BYTES
+from .icp import ICPOdometry, result_from_state
+
+DEFAULT_TILE_PX = 1024
+
+
+class _PlanLevel:
+    def __init__(self, est, levels, tile_px, device):
+        self.est = est
+        self.levels = levels
+        self.params = est.params(levels[0].has_features)
+        arr = (_lib.StbIcpLevel * len(levels))(*[L.struct for L in levels])
+        self.sizes = _lib.StbIcpBatchSizes()
+        lib = _lib.lib()
+        _lib.check(lib.stb_icp_batch_plan(len(levels), arr, tile_px, None, 0,
+                                          ctypes.byref(self.sizes)),
+                   "stb_icp_batch_plan")
+        host = (ctypes.c_uint8 * self.sizes.plan_bytes)()
+        _lib.check(lib.stb_icp_batch_plan(len(levels), arr, tile_px, host,
+                                          self.sizes.plan_bytes,
+                                          ctypes.byref(self.sizes)),
+                   "stb_icp_batch_plan")
+        self.plan = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
+
+
+class BatchedMultiscaleICP:
+    """Registers a list of pairs with the estimators of a
+    ``MultiscaleRegistration`` made of ``ICPOdometry`` levels.
+
+    ``pairs`` is a list of per-pair level lists (coarse first), as returned
+    by ``MultiscaleRegistration.levels`` for each pair.
+    """
+
+    def __init__(self, estimators, pairs, device="cuda:0", tile_px=DEFAULT_TILE_PX,
+                 grid_update=0, grid_corresp=0):
+        if not all(isinstance(est, ICPOdometry) for _, est in estimators):
+            raise TypeError("batched registration needs ICPOdometry estimators")
+        self.device = torch.device(device)
+        self.num_pairs = len(pairs)
+        nlev = len(pairs[0])
+        if any(len(p) != nlev for p in pairs):
+            raise ValueError("every pair needs the same number of levels")
+        self.plans = []
+        for li in range(nlev):
+            est = pairs[0][li][0]
+            plan = _PlanLevel(est, [p[li][1] for p in pairs], tile_px, self.device)
+            plan.params.grid_update = grid_update
+            plan.params.grid_corresp = grid_corresp
+            self.plans.append(plan)
+        self.states = torch.zeros(self.num_pairs, STATE_BYTES, dtype=torch.uint8,
+                                  device=self.device)
+        need = max(p.sizes.scratch_bytes for p in self.plans)
+        self.scratch = torch.empty(need, dtype=torch.uint8, device=self.device)
+        self.t0 = torch.empty(self.num_pairs, 4, 4, dtype=torch.float64,
+                              device=self.device)
+
+    def _args(self, plan):
+        return (_lib.ptr(plan.plan), ctypes.byref(plan.sizes))
+
+    def enqueue(self, transforms=None):
+        """Enqueue every level of every pair (capturable in a hipGraph when
+        ``transforms`` is None or already staged with ``set_initial``)."""
+        lib = _lib.lib()
+        s = _lib.stream()
+        scratch, nbytes = _lib.ptr(self.scratch), self.scratch.numel()
+        states = _lib.ptr(self.states)
+        for i, plan in enumerate(self.plans):
+            t0 = _lib.ptr(self.t0) if (i == 0 and transforms is not None) else None
+            _lib.check(lib.stb_icp_batch_begin(
+                *self._args(plan), states, t0, int(i > 0), scratch, nbytes, s),
+                "stb_icp_batch_begin")
+            _lib.check(lib.stb_icp_batch_iterate(
+                *self._args(plan), ctypes.byref(plan.params), states, scratch,
+                nbytes, plan.est.num_iters, s), "stb_icp_batch_iterate")
+            if i == len(self.plans) - 1 and plan.est._geom_step is not None:
+                _lib.check(lib.stb_icp_batch_information(
+                    *self._args(plan), ctypes.byref(plan.params), states,
+                    scratch, nbytes, s), "stb_icp_batch_information")
+
+    def set_initial(self, transforms):
+        self.t0.copy_(torch.as_tensor(transforms, dtype=torch.float64).view(-1, 4, 4))
+
+    def results(self):
+        """Per-pair RegistrationResult (one host read for the batch)."""
+        raw = self.states.cpu().numpy()
+        last = self.plans[-1]
+        out = []
+        for p in range(self.num_pairs):
+            st = _lib.StbIcpState.from_buffer_copy(raw[p].tobytes())
+            out.append(result_from_state(st, last.levels[p].n,
+                                         last.est._geom_step is not None))
+        return out
+
+    def estimate(self, transforms=None):
+        if transforms is not None:
+            self.set_initial(transforms)
+        self.enqueue(transforms)
+        return self.results()

@@ -280,6 +280,58 @@ int stb_icp_information(const stb_icp_level* level,
                         const stb_icp_params* params, void* workspace,
                         size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Batched Gauss-Newton over many frame pairs of uneven resolution.          */
+/* No reference counterpart: the reference registers one pair per call      */
+/* (slamtb/registration/multiscale.py:30-67); this runs every pair of a      */
+/* batch through the same two persistent launches per iteration.            */
+/* ------------------------------------------------------------------------ */
+
+typedef struct stb_icp_batch_sizes {
+  int32_t num_pairs;
+  int32_t total_tiles;
+  int32_t tile_px;      /* source pixels per tile (multiple of 256) */
+  int32_t storage;      /* storage dtype shared by every pair */
+  int64_t total_map;    /* sum of target pixels over the pairs */
+  size_t plan_bytes;    /* size of the plan image */
+  size_t scratch_bytes; /* device scratch: two maps per pair, partials, tickets */
+} stb_icp_batch_sizes;
+
+/* Host only. Builds the plan image of one pyramid level for `num_pairs`
+ * pairs (levels[p] as in stb_icp_level, device pointers inside) cut into
+ * tiles of `tile_px` source pixels, and fills `sizes`. With plan_host ==
+ * NULL only `sizes` is filled. The caller copies the image to device memory
+ * and passes that device copy as `plan` below. */
+int stb_icp_batch_plan(int32_t num_pairs, const stb_icp_level* levels,
+                       int32_t tile_px, void* plan_host, size_t plan_capacity,
+                       stb_icp_batch_sizes* sizes);
+
+/* `states` is a device array of num_pairs stb_icp_state. Resets every pair
+ * to transforms0[p] (device [num_pairs, 4, 4] FP64, NULL = identity) or,
+ * with `chain`, to its own last accepted pose; clears the maps. */
+int stb_icp_batch_begin(const void* plan, const stb_icp_batch_sizes* sizes,
+                        void* states, const double* transforms0, int32_t chain,
+                        void* scratch, size_t scratch_bytes, void* stream);
+
+/* num_iters iterations of every pair: per iteration one correspondence and
+ * one Jacobian/reduction launch over all tiles; the workgroup completing a
+ * pair solves and updates that pair. A pair whose factorisation fails stops
+ * while the others continue. */
+int stb_icp_batch_iterate(const void* plan, const stb_icp_batch_sizes* sizes,
+                          const stb_icp_params* params, void* states,
+                          void* scratch, size_t scratch_bytes,
+                          int32_t num_iters, void* stream);
+
+/* Per-pair stb_icp_linearize / stb_icp_information over a batch. */
+int stb_icp_batch_linearize(const void* plan, const stb_icp_batch_sizes* sizes,
+                            const stb_icp_params* params, void* states,
+                            void* scratch, size_t scratch_bytes, void* stream);
+int stb_icp_batch_information(const void* plan,
+                              const stb_icp_batch_sizes* sizes,
+                              const stb_icp_params* params, void* states,
+                              void* scratch, size_t scratch_bytes,
+                              void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,3 +78,29 @@ def test_cpu_tensors_are_rejected():
     m = torch.ones(4, 4, dtype=torch.bool)
     with pytest.raises(RuntimeError, match="gpu tensor"):
         Processing.erode_mask(m, torch.empty_like(m))
+
+
+def test_batch_plan_layout_on_the_host():
+    """stb_icp_batch_plan is host-only: tile and map tables of uneven pairs."""
+    lib = _lib.lib()
+    dims = [(120, 160), (240, 320), (60, 80)]
+    levels = (_lib.StbIcpLevel * 3)()
+    for i, (H, W) in enumerate(dims):
+        fake = ctypes.c_void_p(0x1000 * (i + 1))
+        levels[i] = _lib.StbIcpLevel(fake, fake, fake, None, fake, fake, fake,
+                                     None, H * W, H, W, 0, _lib.STB_F32,
+                                     1.0, 1.0, 0.0, 0.0)
+    sizes = _lib.StbIcpBatchSizes()
+    assert lib.stb_icp_batch_plan(3, levels, 1024, None, 0, ctypes.byref(sizes)) == 0
+    tiles = [-(-H * W // 1024) for H, W in dims]
+    assert sizes.total_tiles == sum(tiles)
+    assert sizes.total_map == sum(H * W for H, W in dims)
+    buf = (ctypes.c_uint8 * sizes.plan_bytes)()
+    assert lib.stb_icp_batch_plan(3, levels, 1024, buf, sizes.plan_bytes,
+                                  ctypes.byref(sizes)) == 0
+    raw = bytes(buf)
+    assert np.frombuffer(raw[:16], np.int32).tolist() == [3, sum(tiles), 1024, _lib.STB_F32]
+    pairs = np.frombuffer(raw[-((sum(tiles) * 4 + 63) // 64 * 64):], np.int32)[:sum(tiles)]
+    assert np.array_equal(pairs, np.repeat(np.arange(3), tiles))
+    # tile_px must be a multiple of the 256-lane workgroup
+    assert lib.stb_icp_batch_plan(3, levels, 100, None, 0, ctypes.byref(sizes)) != 0
