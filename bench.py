@@ -45,6 +45,11 @@ def parse():
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--pairs", type=int, default=64,
+                    help="pairs of the batched workload (0 disables it)")
+    ap.add_argument("--no-bf16", action="store_true",
+                    help="skip the 1280x720 BF16-storage workload")
+    ap.add_argument("--no-tune", action="store_true")
     return ap.parse_args()
 
 
@@ -154,6 +159,174 @@ def cpu_baseline(level, est, seconds):
     return iters / el, iters, el, orc.num_threads()
 
 
+def gpu_pair_levels(ms, pair, dev, storage=None):
+    """Bilateral-filtered frame point clouds and pyramid levels of a pair."""
+    import torch
+    from slamtb.camera import KCamera
+    from slamtb.frame import Frame, FrameInfo, FramePointCloud
+    from slamtb.processing import bilateral_depth_filter
+    kcam = KCamera(torch.from_numpy(pair["kcam"]))
+    fps = []
+    for side in ("src", "tgt"):
+        depth = torch.from_numpy(pair[f"{side}_depth"]).to(dev)
+        depth = bilateral_depth_filter(depth, depth > 0, None, 6, 0.03, 4.5, 1.0)
+        fps.append(FramePointCloud.from_frame(
+            Frame(FrameInfo(kcam, 1.0, 0.0), depth_image=depth), device=dev))
+    feats = [torch.from_numpy(pair[f"{s}_feats"]).to(dev) for s in ("src", "tgt")]
+    return ms.levels(fps[0], fps[1], feats[0], feats[1], storage=storage)
+
+
+def capture(fn):
+    import torch
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        fn()  # run once uncaptured (warms caches, allocations)
+        with torch.cuda.graph(g, stream=side):
+            fn()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    return g
+
+
+def time_graph(g, reps):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def batched_section(seed, dev, npairs, tune):
+    """64 pairs drawn from {160x120, 320x240, 640x480, 1280x720}, 3 levels
+    each (10/5/3 it, geometric + 3-ch features): one batched hipGraph vs the
+    same pairs one after another through the single-pair path (also one
+    hipGraph)."""
+    import numpy as np
+    import torch
+    from slamtb import synthetic, metrics
+    from slamtb.registration import ICPOdometry, MultiscaleRegistration
+    from slamtb.registration.batch import BatchedMultiscaleICP
+    from slamtb.registration._fused import GNRunner
+    from slamtb.registration.icp import result_from_state
+    from slamtb import tuning
+
+    rng = np.random.default_rng(seed + 1000)
+    choices = [(160, 120), (320, 240), (640, 480), (1280, 720)]
+    sizes = [choices[i] for i in rng.integers(0, 4, npairs)]
+    kw = dict(geom_weight=1.0, feat_weight=0.1, distance_threshold=0.1,
+              normals_angle_thresh=math.radians(20), feat_residual_thresh=2.75)
+    ms = MultiscaleRegistration([(1.0, ICPOdometry(10, **kw)),
+                                 (0.5, ICPOdometry(5, **kw)),
+                                 (0.5, ICPOdometry(3, **kw))])
+    pairs, truths = [], []
+    for i, (W, H) in enumerate(sizes):
+        pr = synthetic.make_pair(seed + 100 + i, W, H,
+                                 kcam=synthetic.intrinsics(W, H))
+        pairs.append(gpu_pair_levels(ms, pr, dev))
+        truths.append(pr["T_gt"])
+    log(f"batched: {npairs} pairs prepared")
+    if tune:
+        batch, _ = tuning.tune_batch(ms.estimators, pairs, dev)
+    else:
+        batch = BatchedMultiscaleICP(ms.estimators, pairs, dev)
+    g_batch = capture(batch.enqueue)
+    t_batch = time_graph(g_batch, 5)
+    res_b = batch.results()
+    # Plain: every pair through the single-pair fused path, one after the
+    # other (its own runner per pair so its result stays readable).
+    runners = []
+    for levels in pairs:
+        r = GNRunner(dev)
+        for _, L in levels:
+            r.reserve(L.n, L.height, L.width)
+        runners.append(r)
+
+    def plain():
+        for r, levels in zip(runners, pairs):
+            ms.enqueue_levels(r, levels)
+    g_plain = capture(plain)
+    t_plain = time_graph(g_plain, 3)
+    errs, gt_r, gt_t = [], [], []
+    for i, (r, levels) in enumerate(zip(runners, pairs)):
+        ref = result_from_state(r.read_state(), levels[-1][1].n, True)
+        if res_b[i].transform is None or ref.transform is None:
+            errs.append((float("nan"), float("nan")))
+            continue
+        errs.append(metrics.pose_error(ref.transform, res_b[i].transform))
+        e = metrics.pose_error(truths[i], res_b[i].transform)
+        gt_r.append(e[0])
+        gt_t.append(e[1])
+    errs = np.array(errs)
+    return {
+        "pairs": npairs, "tile_px": batch.plans[0].sizes.tile_px,
+        "grid": batch.plans[0].params.grid_update,
+        "batched_pairs_per_s": npairs / t_batch,
+        "plain_pairs_per_s": npairs / t_plain,
+        "speedup_vs_plain": t_plain / t_batch,
+        "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3,
+        "max_pose_err_vs_single_pair": [float(np.nanmax(errs[:, 0])),
+                                         float(np.nanmax(errs[:, 1]))],
+        "median_pose_err_vs_gt": [float(np.median(gt_r)), float(np.median(gt_t))],
+        "resolutions": {f"{W}x{H}": sizes.count((W, H)) for W, H in choices},
+    }
+
+
+def bf16_section(seed, dev, with_oracle):
+    """1280x720, 4 levels (10/5/3/2 it), geometric + 32-channel unit
+    descriptors; BF16 storage against FP32 storage of the same scene."""
+    import numpy as np
+    import torch
+    from slamtb import synthetic, metrics
+    from slamtb.registration import ICPOdometry, MultiscaleRegistration
+    from slamtb.registration._fused import GNRunner
+    from slamtb.registration.icp import result_from_state
+
+    kw = dict(geom_weight=1.0, feat_weight=0.1, distance_threshold=0.1,
+              normals_angle_thresh=math.radians(20), feat_residual_thresh=2.75)
+    ms = MultiscaleRegistration([(1.0, ICPOdometry(10, **kw)),
+                                 (0.5, ICPOdometry(5, **kw)),
+                                 (0.5, ICPOdometry(3, **kw)),
+                                 (0.5, ICPOdometry(2, **kw))])
+    pr = synthetic.make_pair(seed + 500, 1280, 720,
+                             kcam=synthetic.intrinsics(1280, 720), channels=32,
+                             descriptor=True)
+    out = {}
+    for name, storage in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
+        levels = gpu_pair_levels(ms, pr, dev, storage=storage)
+        est, top = levels[-1]
+        params = est.params(True)
+        run = GNRunner(dev)
+        for _, L in levels:
+            run.reserve(L.n, L.height, L.width)
+        K = 100
+        run.begin(top)
+        g = capture(lambda: run.iterate(top, params, K))
+        run.begin(top)
+        t = time_graph(g, 1) / K
+        s, C = (2 if storage == torch.bfloat16 else 4), 32
+        bytes_it = top.n * (12 * s + 18 + 9 * s + 9 + 2 * C * s)
+        g_reg = capture(lambda: ms.enqueue_levels(run, levels))
+        t_reg = time_graph(g_reg, 3)
+        res = result_from_state(run.read_state(), top.n, True)
+        entry = {"iters_per_s": 1.0 / t, "ms_per_iter": t * 1e3,
+                 "gbs_all_pixels": bytes_it / t / 1e9,
+                 "registration_ms": t_reg * 1e3,
+                 "pose_err_vs_gt": list(metrics.pose_error(pr["T_gt"], res.transform))}
+        if with_oracle and name == "bf16":
+            ref = oracle_levels([(e, L) for e, L in levels])
+            entry["pose_err_vs_oracle"] = list(metrics.pose_error(ref["transform"],
+                                                                  res.transform))
+        out[name] = entry
+        out[name + "_transform"] = res.transform
+    out["pose_diff_bf16_vs_fp32"] = list(metrics.pose_error(out.pop("fp32_transform"),
+                                                            out.pop("bf16_transform")))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -260,6 +433,15 @@ def main():
         ref = oracle_levels(levels)
         err_or = metrics.pose_error(ref["transform"], T_gpu)
 
+    batched = None
+    if args.pairs > 0:
+        log(f"batched workload: {args.pairs} pairs")
+        batched = batched_section(args.seed, dev, args.pairs, not args.no_tune)
+    bf16 = None
+    if not args.no_bf16:
+        log("1280x720 BF16 / FP32 storage workload")
+        bf16 = bf16_section(args.seed, dev, not args.no_oracle)
+
     cpu = None
     if not args.no_cpu_baseline:
         log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")
@@ -291,6 +473,8 @@ def main():
         "registration_ms_per_pair": reg_ms,
         "pose_err_vs_oracle": err_or, "pose_err_vs_gt": list(err_gt),
         "match_ratio": result.match_ratio,
+        "batched": batched,
+        "bf16_1280x720": bf16,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -94,9 +94,14 @@ __device__ inline void corresp_pixel(const LevelView<S>& L, const float* m,
                                      unsigned long long* __restrict__ map,
                                      float dthr2, float cstar) {
 #pragma clang fp contract(off)
-  if (!L.sm[idx]) return;
-  const V3<float> p = rigid_point(m, ld3f(L.sp, idx));
-  const V3<float> nn = rigid_normal(nm, ld3f(L.sn, idx));
+  // Mask, point and normal loads issue together; idx < n keeps them in
+  // bounds whatever the mask says.
+  const uint8_t valid_src = L.sm[idx];
+  const V3<float> ps = ld3f(L.sp, idx);
+  const V3<float> ns = ld3f(L.sn, idx);
+  if (!valid_src) return;
+  const V3<float> p = rigid_point(m, ps);
+  const V3<float> nn = rigid_normal(nm, ns);
   float u, v;
   project(L.fx, L.fy, L.cx, L.cy, p, u, v);
   int ui, vi;
@@ -141,17 +146,25 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
                                       int64_t idx, const TermWeights& tw,
                                       float (&acc)[kSlots]) {
 #pragma clang fp contract(off)
-  if (!L.sm[idx]) return;
-  const V3<float> p = rigid_point(m, ld3f(L.sp, idx));
+  const uint8_t valid_src = L.sm[idx];
+  const V3<float> ps = ld3f(L.sp, idx);
+  if (!valid_src) return;
+  const V3<float> p = rigid_point(m, ps);
   float u, v;
   project(L.fx, L.fy, L.cx, L.cy, p, u, v);
   int ui, vi;
   if (!round_in_range(u, L.W, ui) || !round_in_range(v, L.H, vi)) return;
   const int64_t pos = int64_t(vi) * L.W + ui;
-  if (uint32_t(map[pos]) != uint32_t(idx)) return;
+  // The owner word and the target point/normal sit at the same in-range
+  // pixel: load them together so the owner test does not add a round trip.
+  const uint32_t owner = uint32_t(map[pos]);
+  V3<float> q, qn;
   if (GEOM) {
-    const V3<float> q = ld3f(L.tp, pos);
-    const V3<float> qn = ld3f(L.tn, pos);
+    q = ld3f(L.tp, pos);
+    qn = ld3f(L.tn, pos);
+  }
+  if (owner != uint32_t(idx)) return;
+  if (GEOM) {
     const float r = dot3(sub3(q, p), qn);
     const V3<float> c = cross3(p, qn);
     const float j[6] = {qn.x, qn.y, qn.z, c.x, c.y, c.z};
