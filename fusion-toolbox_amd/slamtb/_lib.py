@@ -145,6 +145,8 @@ _SIGNATURES = {
         _vp, _BATCH, _PARAMS, _vp, _vp, _sz, _vp]),
     "stb_icp_batch_information": (ctypes.c_int, [
         _vp, _BATCH, _PARAMS, _vp, _vp, _sz, _vp]),
+    "stb_icp_batch_adjoint": (ctypes.c_int, [
+        _vp, _BATCH, _PARAMS, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

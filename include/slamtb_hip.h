@@ -332,6 +332,35 @@ int stb_icp_batch_information(const void* plan,
                               void* scratch, size_t scratch_bytes,
                               void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Adjoint of one Gauss-Newton step (differentiable ICP). No reference       */
+/* counterpart; the reference's differentiable registration is AutogradICP   */
+/* (slamtb/registration/autogradicp.py:95-374), built on the same            */
+/* correspondence and exp_rt backward ops.                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Per-pair gradient buffers (device pointers, FP32, accumulated into). */
+typedef struct stb_icp_grads {
+  float* d_src_points;  /* [n_src, 3] */
+  float* d_src_feats;   /* [channels, n_src] */
+  float* d_tgt_points;  /* [height * width, 3] */
+  float* d_tgt_normals; /* [height * width, 3] */
+  float* d_tgt_feats;   /* [channels, height * width] */
+} stb_icp_grads;
+
+/* With every pair's pose reset to T_k (stb_icp_batch_begin), rebuilds the
+ * correspondences at T_k and, for each pair p with coef[p] = [lambda (6);
+ * x (6)] (lambda = A_k^-1 dL/dx_k, x = the step's solution), adds the
+ * pixels' contributions to grads[p] and writes dL/dT_k (3x4 rows, the part
+ * that flows through the transformed source points) to dT[p] (12 doubles,
+ * fixed-order FP64 sum). FP32 storage only. `grads` is a device array of
+ * num_pairs structs. */
+int stb_icp_batch_adjoint(const void* plan, const stb_icp_batch_sizes* sizes,
+                          const stb_icp_params* params, void* states,
+                          const stb_icp_grads* grads, const double* coef,
+                          double* dT, void* scratch, size_t scratch_bytes,
+                          void* stream);
+
 #ifdef __cplusplus
 }
 #endif

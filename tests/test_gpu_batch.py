@@ -17,7 +17,7 @@ from slamtb.registration.batch import BatchedMultiscaleICP
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
 
-KW = dict(geom_weight=1.0, feat_weight=0.1, distance_threshold=0.1,
+KW = dict(geom_weight=10.0, feat_weight=1.0, distance_threshold=0.1,
           normals_angle_thresh=math.radians(20), feat_residual_thresh=2.75)
 
 

@@ -238,9 +238,9 @@ def test_icp_config1_matches_oracle(cfg1):
 
 def test_icp_features_matches_oracle():
     P = prepared_pair(5, 320, 240, channels=3)
-    kw = dict(geom_weight=1.0, feat_weight=0.1, distance_threshold=0.1,
+    kw = dict(geom_weight=10.0, feat_weight=1.0, distance_threshold=0.1,
               normals_angle_thresh=ANG20, feat_residual_thresh=2.75)
-    res = run_icp(P, 5, gw=1.0, fw=0.1)
+    res = run_icp(P, 5, gw=10.0, fw=1.0)
     ref = orc.icp_estimate(P["kcam"], P["src_pts"], P["src_nrm"], P["src_mask"],
                            P["tgt_pts"], P["tgt_nrm"], P["tgt_mask"], 5,
                            src_feats=P["src_feats"], tgt_feats=P["tgt_feats"], **kw)

@@ -118,7 +118,7 @@ def test_multiscale_registration_matches_oracle():
                                                6, 0.03, 4.5, 1.0)
         fps.append(FramePointCloud.from_frame(Frame(FrameInfo(kcam), depth_image=dd),
                                               device=DEV))
-    kw = dict(geom_weight=1.0, feat_weight=0.1, distance_threshold=0.1,
+    kw = dict(geom_weight=10.0, feat_weight=1.0, distance_threshold=0.1,
               normals_angle_thresh=math.radians(20), feat_residual_thresh=2.75)
     ms = MultiscaleRegistration([(1.0, ICPOdometry(10, **kw)),
                                  (0.5, ICPOdometry(5, **kw)),
