@@ -53,7 +53,32 @@ def parse():
     ap.add_argument("--no-bf16", action="store_true",
                     help="skip the 1280x720 BF16-storage workload")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="only the 640x480 steps and the per-kernel timing "
+                         "(for profiler runs: every launch is the same kernel "
+                         "at the same size)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="copy a MIB-MiB device buffer 10 times and exit (the "
+                         "reference reading for the HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy of known size: what the HBM byte
+    counters read for it scales the counters of the ICP kernels."""
+    import torch
+    n = mib << 20
+    a = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 10
+    print(json.dumps({"copy_probe_bytes": n, "read_bytes": n, "write_bytes": n,
+                      "gbs": 2 * n / dt / 1e9}), flush=True)
 
 
 def log(msg):
@@ -330,18 +355,52 @@ def bf16_section(seed, dev, with_oracle):
     return out
 
 
+def registration_section(args, ms, levels, run, top, pair):
+    """The full 3-level registration of the pair captured in one hipGraph:
+    ms per pair, the result and its pose error against the ground truth and
+    the oracle fed the same level inputs."""
+    import torch
+    from slamtb import metrics
+    from slamtb.registration.icp import result_from_state
+    log("full 3-level registration (graph)")
+    reg_graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        ms.enqueue_levels(run, levels)  # warm the sequence once uncaptured
+        with torch.cuda.graph(reg_graph, stream=side):
+            ms.enqueue_levels(run, levels)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    reps_reg = 50
+    t0 = time.perf_counter()
+    for _ in range(reps_reg):
+        reg_graph.replay()
+    torch.cuda.synchronize()
+    reg_ms = (time.perf_counter() - t0) * 1e3 / reps_reg
+    result = result_from_state(run.read_state(), top.n, True)
+    T_gpu = result.transform.numpy()
+    err_gt = metrics.pose_error(pair["T_gt"], T_gpu)
+    err_or = None
+    if not args.no_oracle:
+        log("oracle registration on the same inputs")
+        ref = oracle_levels(levels)
+        err_or = metrics.pose_error(ref["transform"], T_gpu)
+    return reg_ms, result, list(err_gt), err_or
+
+
 def main():
     args = parse()
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not "
               "supported (independent frame pairs shard with no exchange; "
               "see the multi-GPU note in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
     from slamtb.registration._fused import runner_for
-    from slamtb.registration.icp import result_from_state
-    from slamtb import metrics
 
     dev = torch.device("cuda:0")
     log("preparing 640x480 pair on the GPU")
@@ -388,7 +447,7 @@ def main():
     log("per-kernel event timing")
     run.begin(top)
     run.iterate(top, params, 2)
-    reps = 200
+    reps = max(10, min(200, K)) if args.profile else 200
     (a_avg, a_med), (b_only, b_med) = time_iteration_kernels(run, top, params, reps)
     st2 = run.read_state()
     n = top.n
@@ -410,31 +469,13 @@ def main():
         dom = dict(kernel="jacobian_fused_kernel", ms=b_only, gbs=b_gbs,
                    bytes=bytes_b, bytes_all=bytes_b_all)
 
-    # --- full 3-level registration -----------------------------------------
-    log("full 3-level registration (graph)")
-    reg_graph = torch.cuda.CUDAGraph()
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        ms.enqueue_levels(run, levels)  # warm the sequence once uncaptured
-        with torch.cuda.graph(reg_graph, stream=side):
-            ms.enqueue_levels(run, levels)
-    torch.cuda.current_stream().wait_stream(side)
-    torch.cuda.synchronize()
-    reps_reg = 50
-    t0 = time.perf_counter()
-    for _ in range(reps_reg):
-        reg_graph.replay()
-    torch.cuda.synchronize()
-    reg_ms = (time.perf_counter() - t0) * 1e3 / reps_reg
-    result = result_from_state(run.read_state(), top.n, True)
-    T_gpu = result.transform.numpy()
-    err_gt = metrics.pose_error(pair["T_gt"], T_gpu)
-    err_or = None
-    if not args.no_oracle:
-        log("oracle registration on the same inputs")
-        ref = oracle_levels(levels)
-        err_or = metrics.pose_error(ref["transform"], T_gpu)
+    if args.profile:
+        args.no_oracle = args.no_cpu_baseline = args.no_bf16 = True
+        args.pairs = 0
+    reg_ms, result, err_gt, err_or = None, None, None, None
+    if not args.profile:
+        reg_ms, result, err_gt, err_or = registration_section(
+            args, ms, levels, run, top, pair)
 
     batched = None
     if args.pairs > 0:
@@ -474,8 +515,8 @@ def main():
         "kernels_ms": {"corresp_fused_kernel": a_avg, "jacobian_fused_kernel": b_only,
                        "corresp_median": a_med, "jacobian_median": b_med},
         "registration_ms_per_pair": reg_ms,
-        "pose_err_vs_oracle": err_or, "pose_err_vs_gt": list(err_gt),
-        "match_ratio": result.match_ratio,
+        "pose_err_vs_oracle": err_or, "pose_err_vs_gt": err_gt,
+        "match_ratio": None if result is None else result.match_ratio,
         "batched": batched,
         "bf16_1280x720": bf16,
         "cpu_baseline": cpu,

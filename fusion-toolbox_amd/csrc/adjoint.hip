@@ -11,13 +11,16 @@
 // forward), pushes J_bar / r_bar through the point-to-plane and feature terms
 // into the source points, target points and normals, source and target
 // feature maps, and reduces dL/dT_k (the 3x4 pose rows, through
-// p' = R p + t) per pair in FP64. The caller supplies lambda and x per pair
-// and resets each pair's pose to T_k (stb_icp_batch_begin) before the call;
-// correspondences are rebuilt at T_k exactly as the forward built them.
+// p' = R p + t) per pair, accumulated in FP64 from the first product. The
+// caller supplies lambda and x per pair and resets each pair's pose to T_k
+// (stb_icp_batch_begin) before the call; correspondences are rebuilt at T_k
+// exactly as the forward built them.
 #include "batch_plan.hpp"
 
 namespace stb {
 namespace {
+
+constexpr int kPose = 12;  // dL/dT rows 0..2 of the 4x4 pose, row-major
 
 __device__ inline float dot6(const float* a, const float* b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3] + a[4] * b[4] +
@@ -66,6 +69,32 @@ __device__ inline void bil_scatter(float* __restrict__ ch, int W, const BilD& b,
   if (b.rx && b.ry) atomicAdd(ch + o + W + 1, g * b.ur * b.vr);
 }
 
+// Block sum of the 12 pose-gradient terms in FP64 (lanes by shuffles, waves
+// through LDS), written as one kSlots-wide partial (unused slots zero) for
+// the fixed-order sum of the last block.
+__device__ inline void pose_partial(double (&acc)[kPose], double* out,
+                                    double* sh) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kPose; ++k)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < kPose; ++k) sh[wave * kPose + k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < kSlots) {
+    double s = 0.0;
+    if (threadIdx.x < kPose)
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) s += sh[w * kPose + threadIdx.x];
+    out[threadIdx.x] = s;
+    __threadfence();
+  }
+  __syncthreads();
+}
+
 // J'_bar and r'_bar of one weighted term (see the header comment).
 __device__ inline void term_adjoint(const float J[6], float w, float s, float r,
                                     const float* lam, const float* x,
@@ -87,7 +116,7 @@ __device__ inline void adjoint_pixel(const LevelView<float>& L, const float* m,
                                      int64_t idx, const TermWeights& tw,
                                      const float* lam, const float* x,
                                      const stb_icp_grads& G,
-                                     float (&tacc)[kSlots]) {
+                                     double (&tacc)[kPose]) {
   if (!L.sm[idx]) return;
   const V3<float> ps = ld3f(L.sp, idx);
   const V3<float> p = rigid_point(m, ps);
@@ -200,8 +229,8 @@ __device__ inline void adjoint_pixel(const LevelView<float>& L, const float* m,
   sp[0] += m[0] * pb.x + m[4] * pb.y + m[8] * pb.z;
   sp[1] += m[1] * pb.x + m[5] * pb.y + m[9] * pb.z;
   sp[2] += m[2] * pb.x + m[6] * pb.y + m[10] * pb.z;
-  const float pbv[3] = {pb.x, pb.y, pb.z};
-  const float psv[3] = {ps.x, ps.y, ps.z};
+  const double pbv[3] = {pb.x, pb.y, pb.z};
+  const double psv[3] = {ps.x, ps.y, ps.z};
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
 #pragma unroll
@@ -242,15 +271,15 @@ __global__ void __launch_bounds__(kBlock)
     const int first = pv.tile_start[p];
     const int ntiles = pv.tile_start[p + 1] - first;
     const int64_t base = int64_t(t - first) * pv.tile_px;
-    float tacc[kSlots];
+    double tacc[kPose];
 #pragma unroll
-    for (int k = 0; k < kSlots; ++k) tacc[k] = 0.0f;
+    for (int k = 0; k < kPose; ++k) tacc[k] = 0.0;
     for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
       const int64_t idx = base + k;
       if (idx < L.n)
         adjoint_pixel<GEOM, FEAT>(L, m, map, idx, tw, lam, x, G, tacc);
     }
-    block_partial(tacc, partials + int64_t(t) * kSlots, sh);
+    pose_partial(tacc, partials + int64_t(t) * kSlots, sh);
     if (arrive_last(&tickets[p], ntiles - 1, &s_last)) {
       sum_partials(partials + int64_t(first) * kSlots, ntiles, sh, tot);
       if (threadIdx.x < 12) dT[12 * p + threadIdx.x] = tot[threadIdx.x];

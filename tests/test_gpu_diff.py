@@ -1,13 +1,15 @@
-"""Differentiable ICP (unrolled GN with the hand-written adjoint).
+"""Differentiable ICP (unrolled GN with the hand-written adjoint) against
+torch autograd through the pure-PyTorch FP64 restatement
+(oracle/torch_gn.py, itself pinned on the CPU against the oracle's GN poses
+and central finite differences).
 
-Checks, for a batch of pairs and 5 geometric + 3-channel feature steps:
-* the adjoint's gradients of the pose loss with respect to the source depth,
-  target points and normals, source and target features and the initial
-  6-DoF pose equal torch autograd through the pure-PyTorch FP64 restatement
-  (oracle/torch_gn.py) driven with the same correspondences (norm-wise
-  relative error <= 2e-3: the adjoint runs in FP32);
-* that restatement's gradients equal FP64 central finite differences
-  (eps = 1e-4) on 64 random source pixels and all 6 pose dimensions.
+For a batch of two pairs and 5 geometric + 3-channel feature steps, the
+restatement is driven with the correspondences and feature gate the GPU
+forward made at its own poses (the same drop-in kernels at the same FP32
+pose), so both sides differentiate the same piecewise-smooth function. The
+adjoint runs in FP32 per pixel: gradients with respect to the source depth,
+target points and normals, source and target features and the initial 6-DoF
+pose must agree with autograd to a norm-wise relative error of 2e-3.
 """
 import math
 
@@ -27,30 +29,30 @@ DEV = torch.device("cuda:0")
 CFG = dict(geom_weight=10.0, feat_weight=1.0, distance_threshold=0.1,
            normals_angle_thresh=math.radians(20), feat_residual_thresh=2.75)
 STEPS = 5
+NONE = 0x0FFFFFFF7F800000
 
 
-def decisions(P, T, dev=DEV):
-    """Owners and feature gate of one step at pose T (the GPU forward's own
-    rules, through the drop-in kernels)."""
-    g = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+def decisions(P, T):
+    """Owners (source index, target position) and feature gate of one step at
+    pose T, through the drop-in kernels."""
+    g = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
     k = g(P["kcam"])
-    rt = T.float().to(dev)
+    rt = T.float().to(DEV)
     cmap = CorrespondenceMap(CFG["distance_threshold"], CFG["normals_angle_thresh"])(
         g(P["src_pts"]).view(-1, 3), g(P["src_nrm"]).view(-1, 3),
         g(P["src_mask"]).view(-1), rt, g(P["tgt_pts"]), g(P["tgt_nrm"]),
         g(P["tgt_mask"]), k)
     flat = cmap.view(-1)
-    pos = torch.nonzero(flat != 0x0FFFFFFF7F800000).view(-1)
-    src = (flat[pos] >> 32)
+    pos = torch.nonzero(flat != NONE).view(-1)
+    src = flat[pos] >> 32
     n = flat.numel()
-    jtj = torch.empty(n, 6, 6, device=dev)
-    jr = torch.empty(n, 6, device=dev)
-    r2 = torch.empty(n, device=dev)
+    jtj = torch.empty(n, 6, 6, device=DEV)
+    jr = torch.empty(n, 6, device=DEV)
+    r2 = torch.empty(n, device=DEV)
     ICPJacobian.estimate_feature(g(P["src_pts"]).view(-1, 3), g(P["src_feats"]).view(3, -1),
                                  g(P["src_mask"]).view(-1), rt, g(P["tgt_feats"]), k,
                                  cmap, CFG["feat_residual_thresh"], jtj, jr, r2)
-    ok = r2[src] > 0
-    return src.cpu(), pos.cpu(), ok.cpu()
+    return src.cpu(), pos.cpu(), (r2[src] > 0).cpu()
 
 
 def backproject(depth, kcam):
@@ -69,52 +71,42 @@ def reference_inputs(P):
                 k=d(P["kcam"]))
 
 
-def reference_steps(P, R, x0):
-    steps = []
-    with torch.no_grad():
-        for _ in range(STEPS):
-            T = torch_gn.unrolled(backproject(R["depth"], R["k"]), R["tp"], R["tn"],
-                                  R["sf"], R["tf"], R["k"], x0, steps,
-                                  CFG["geom_weight"], CFG["feat_weight"])
-            steps.append(decisions(P, T))
-    return steps
-
-
-def reference_loss(R, steps, x0, T_gt, depth=None, tp=None, tn=None, sf=None,
-                   tf=None):
-    depth = R["depth"] if depth is None else depth
-    T = torch_gn.unrolled(backproject(depth, R["k"]),
-                          R["tp"] if tp is None else tp,
-                          R["tn"] if tn is None else tn,
-                          R["sf"] if sf is None else sf,
-                          R["tf"] if tf is None else tf, R["k"], x0, steps,
-                          CFG["geom_weight"], CFG["feat_weight"])
-    return pose_loss(T, T_gt)
+def reference_pose(R, steps, x0, depth=None, tp=None, tn=None, sf=None, tf=None):
+    pick = lambda v, key: R[key] if v is None else v
+    return torch_gn.unrolled(backproject(pick(depth, "depth"), R["k"]),
+                             pick(tp, "tp"), pick(tn, "tn"), pick(sf, "sf"),
+                             pick(tf, "tf"), R["k"], x0, steps,
+                             CFG["geom_weight"], CFG["feat_weight"])
 
 
 @pytest.fixture(scope="module")
 def batch():
     pairs = [prepared_pair(70 + i, 160, 120, channels=3) for i in range(2)]
     rng = np.random.default_rng(5)
-    x0 = rng.normal(size=(2, 6)) * np.array([0.01] * 3 + [0.01] * 3)
+    x0 = rng.normal(size=(2, 6)) * 0.01
     return pairs, x0
 
 
-def gpu_grads(pairs, x0):
-    st = lambda key, dt=torch.float32: torch.stack(
-        [torch.from_numpy(np.ascontiguousarray(p[key])) for p in pairs]).to(DEV, dt)
-    depth = st("src_depth").requires_grad_()
-    tp = st("tgt_pts").requires_grad_()
-    tn = st("tgt_nrm").requires_grad_()
-    sf = st("src_feats").requires_grad_()
-    tf = st("tgt_feats").requires_grad_()
-    x = torch.from_numpy(x0).to(DEV).requires_grad_()
+def stacked(pairs, key, dt=torch.float32):
+    return torch.stack([torch.from_numpy(np.ascontiguousarray(p[key]))
+                        for p in pairs]).to(DEV, dt)
+
+
+def run_gpu(pairs, x0, steps, grad=False):
+    depth = stacked(pairs, "src_depth").requires_grad_(grad)
+    tp = stacked(pairs, "tgt_pts").requires_grad_(grad)
+    tn = stacked(pairs, "tgt_nrm").requires_grad_(grad)
+    sf = stacked(pairs, "src_feats").requires_grad_(grad)
+    tf = stacked(pairs, "tgt_feats").requires_grad_(grad)
+    x = torch.from_numpy(x0).to(DEV).requires_grad_(grad)
+    icp = DifferentiableICP(steps, **CFG)
+    T = icp(pairs[0]["kcam"], depth, stacked(pairs, "src_nrm"),
+            stacked(pairs, "src_mask", torch.bool), tp, tn,
+            stacked(pairs, "tgt_mask", torch.bool), x, src_feats=sf, tgt_feats=tf)
+    if not grad:
+        return T.detach().cpu(), None
     T_gt = torch.from_numpy(np.stack([p["T_gt"] for p in pairs])).to(DEV)
-    icp = DifferentiableICP(STEPS, **CFG)
-    T = icp(pairs[0]["kcam"], depth, st("src_nrm"), st("src_mask", torch.bool), tp, tn,
-            st("tgt_mask", torch.bool), x, src_feats=sf, tgt_feats=tf)
-    loss = pose_loss(T, T_gt).sum()
-    loss.backward()
+    pose_loss(T, T_gt).sum().backward()
     return T.detach().cpu(), dict(depth=depth.grad.cpu(), tp=tp.grad.cpu(),
                                   tn=tn.grad.cpu(), sf=sf.grad.cpu(),
                                   tf=tf.grad.cpu(), x0=x.grad.cpu())
@@ -127,16 +119,20 @@ def rel(a, b):
 
 def test_adjoint_matches_autograd_of_the_reference(batch):
     pairs, x0 = batch
-    T_gpu, G = gpu_grads(pairs, x0)
+    # The GPU forward's own poses T_0 .. T_4 and the decisions taken there.
+    poses = [run_gpu(pairs, x0, k)[0] for k in range(STEPS)]
+    T_gpu, G = run_gpu(pairs, x0, STEPS, grad=True)
+    errors = []
     for b, P in enumerate(pairs):
+        steps = [decisions(P, poses[k][b]) for k in range(STEPS)]
         R = reference_inputs(P)
-        xr = torch.from_numpy(x0[b]).clone()
-        steps = reference_steps(P, R, xr)
         leaves = {k: R[k].clone().requires_grad_() for k in ("depth", "tp", "tn", "sf", "tf")}
-        xg = xr.clone().requires_grad_()
-        L = reference_loss(R, steps, xg, torch.from_numpy(P["T_gt"]),
-                           **{k: v for k, v in leaves.items()})
-        L.backward()
+        xg = torch.from_numpy(x0[b]).clone().requires_grad_()
+        T_ref = reference_pose(R, steps, xg, **leaves)
+        fwd = float((T_ref.detach() - T_gpu[b]).abs().max())
+        print(f"pair {b}: matched {[len(s[0]) for s in steps]}, |T_gpu - T_ref| {fwd:.2e}")
+        assert fwd < 1e-5
+        pose_loss(T_ref, torch.from_numpy(P["T_gt"])).backward()
         ref = {k: v.grad.numpy() for k, v in leaves.items()}
         ref["x0"] = xg.grad.numpy()
         got = {"depth": G["depth"][b].numpy(), "tp": G["tp"][b].numpy(),
@@ -145,37 +141,6 @@ def test_adjoint_matches_autograd_of_the_reference(batch):
         for key in ref:
             e = rel(got[key].reshape(-1), ref[key].reshape(-1))
             print(f"pair {b} d/d{key}: rel err {e:.2e} (|ref| {np.linalg.norm(ref[key]):.3e})")
-            assert e < 2e-3, (b, key, e)
-
-
-def test_reference_gradients_match_central_differences(batch):
-    pairs, x0 = batch
-    P = pairs[0]
-    R = reference_inputs(P)
-    xr = torch.from_numpy(x0[0]).clone()
-    steps = reference_steps(P, R, xr)
-    T_gt = torch.from_numpy(P["T_gt"])
-    depth = R["depth"].clone().requires_grad_()
-    xg = xr.clone().requires_grad_()
-    reference_loss(R, steps, xg, T_gt, depth=depth).backward()
-    eps = 1e-4
-    rng = np.random.default_rng(0)
-    valid = np.flatnonzero(P["src_mask"].reshape(-1))
-    pix = rng.choice(valid, 64, replace=False)
-    fd = []
-    with torch.no_grad():
-        for p in pix:
-            dp, dm = R["depth"].clone(), R["depth"].clone()
-            dp.view(-1)[p] += eps
-            dm.view(-1)[p] -= eps
-            fd.append((reference_loss(R, steps, xr, T_gt, depth=dp) -
-                       reference_loss(R, steps, xr, T_gt, depth=dm)).item() / (2 * eps))
-        fdx = []
-        for i in range(6):
-            e = torch.zeros(6, dtype=torch.float64)
-            e[i] = eps
-            fdx.append((reference_loss(R, steps, xr + e, T_gt) -
-                        reference_loss(R, steps, xr - e, T_gt)).item() / (2 * eps))
-    ad = depth.grad.view(-1)[pix].numpy()
-    assert rel(ad, fd) < 1e-3, rel(ad, fd)
-    assert rel(xg.grad.numpy(), fdx) < 1e-3, rel(xg.grad.numpy(), fdx)
+            errors.append((b, key, e))
+    bad = [e for e in errors if not e[2] < 2e-3]
+    assert not bad, bad

@@ -2,9 +2,12 @@
 the oracle (through the C ABI via the drop-in shim).
 
 Bar: integer/bool outputs exact; normals, downsampling and the blur
-bit-equal (same IEEE operation order); the bilateral filter within 4 ulp
-(its exp differs between the device library and the host libm); SE3 maps
-within 1e-12 (FP64 sin/cos differ by an ulp between libraries).
+bit-equal (same IEEE operation order); the bilateral filter within 16 ulp:
+its weights come from expf, which differs by up to an ulp between the device
+library and the host libm, and each output is a ratio of two FP32 sums over
+up to 13x13 taps, where a one-ulp change of a weight can change the rounding
+of every later partial sum (5 ulp seen at 320x240); SE3 maps within 1e-12
+(FP64 sin/cos differ by an ulp between libraries).
 """
 import math
 
@@ -37,7 +40,7 @@ def test_bilateral_float_and_int(pair):
     d = pair["src_depth"]
     got = processing.bilateral_depth_filter(g(d), g(d > 0), None, 6, 0.03, 4.5, 1.0)
     ref = orc.bilateral(d, d > 0, 6, 0.03, 4.5, 1.0)
-    np.testing.assert_array_max_ulp(got.cpu().numpy(), ref, maxulp=4)
+    np.testing.assert_array_max_ulp(got.cpu().numpy(), ref, maxulp=16)
     mm = (d * 1000).astype(np.int32)
     got = processing.bilateral_depth_filter(g(mm), g(mm > 0), None, 13,
                                             29.9999880000072, 4.50000000225)
