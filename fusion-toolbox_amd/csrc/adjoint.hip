@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     pose_partial(tacc, partials + int64_t(t) * kSlots, sh);
     if (arrive_last(&tickets[p], ntiles - 1, &s_last)) {
-      sum_partials(partials + int64_t(first) * kSlots, ntiles, sh, tot);
+      sum_partials<kBlock>(partials + int64_t(first) * kSlots, ntiles, sh, tot);
       if (threadIdx.x < 12) dT[12 * p + threadIdx.x] = tot[threadIdx.x];
       if (threadIdx.x == 0) tickets[p] = 0;
     }

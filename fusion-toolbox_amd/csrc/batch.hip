@@ -2,15 +2,14 @@
 //
 // A batch plan (built on the host by stb_icp_batch_plan, uploaded by the
 // caller) lists one level descriptor per pair and cuts every pair's source
-// pixels into equal tiles. Each iteration is two persistent launches over the
-// flattened tile list of all pairs -- correspondence, then Jacobians and
-// reduction -- whatever the pair count or resolutions. A workgroup walks
-// tiles t = blockIdx.x, blockIdx.x + gridDim.x, ...; for every Jacobian tile
-// it writes one FP64 partial and takes an arrival ticket on its pair; the
-// workgroup that completes a pair sums that pair's partials in tile order,
-// solves the pair's 6x6 system in FP64 and updates that pair's pose. All
-// pairs therefore advance one iteration per launch pair, with no host
-// involvement, and a whole multi-level loop can be captured in one hipGraph.
+// pixels into equal tiles. Each iteration is three launches whatever the pair
+// count or resolutions: correspondence and Jacobians are persistent launches
+// over the flattened tile list of all pairs (a workgroup walks tiles
+// t = blockIdx.x, blockIdx.x + gridDim.x, ...; every Jacobian tile leaves one
+// FP64 partial), then one workgroup per pair sums that pair's partials in
+// tile order, solves its 6x6 system in FP64 and updates its pose. All pairs
+// advance one iteration per three launches with no host involvement, and a
+// whole multi-level loop can be captured in one hipGraph.
 #include "batch_plan.hpp"
 
 #include <cstring>
@@ -53,12 +52,9 @@ __global__ void __launch_bounds__(kBlock)
                           stb_icp_state* __restrict__ states,
                           unsigned long long* __restrict__ maps0,
                           unsigned long long* __restrict__ maps1,
-                          double* __restrict__ partials,
-                          int32_t* __restrict__ tickets, TermWeights tw,
-                          int do_solve, int fixed_map) {
+                          double* __restrict__ partials, TermWeights tw,
+                          int fixed_map) {
   __shared__ double sh[kShared];
-  __shared__ double tot[kSlots];
-  __shared__ int s_last;
   const PlanView pv = plan_view(plan);
   for (int t = blockIdx.x; t < pv.tiles; t += gridDim.x) {
     const int p = pv.tile_pair[t];
@@ -91,19 +87,27 @@ __global__ void __launch_bounds__(kBlock)
       const int64_t idx = base + k;
       if (idx < L.n) jacobian_pixel<S, GEOM, FEAT>(L, m, map, idx, tw, acc);
     }
-    block_partial(acc, partials + int64_t(t) * kSlots, sh);
-    if (arrive_last(&tickets[p], ntiles - 1, &s_last)) {
-      sum_partials(partials + int64_t(first) * kSlots, ntiles, sh, tot);
-      if (threadIdx.x == 0) {
-        tickets[p] = 0;
-        if (do_solve)
-          gn_finalize(st, tot, tw.gw, tw.fw, sh);
-        else
-          store_linearization(st, tot);
-      }
-    }
-    __syncthreads();  // LDS is reused by the next tile
+    block_partial(acc, partials + int64_t(t) * kSlots, sh, false);
   }
+}
+
+// One block per pair: the fixed-order sum of the pair's tile partials, then
+// its pose update (do_solve) or stored linearisation.
+__global__ void __launch_bounds__(kSolveBlock)
+    batch_solve_kernel(const void* __restrict__ plan,
+                       stb_icp_state* __restrict__ states,
+                       const double* __restrict__ partials, double gw,
+                       double fw, int do_solve, int fixed_map) {
+  __shared__ double sh[kSolveBlock];
+  __shared__ double tot[kSlots];
+  const PlanView pv = plan_view(plan);
+  const int p = blockIdx.x;
+  if (p >= pv.P) return;
+  stb_icp_state* st = states + p;
+  if (!fixed_map && st->stopped) return;
+  const int first = pv.tile_start[p];
+  finish_partials<kSolveBlock>(st, partials + int64_t(first) * kSlots,
+                  pv.tile_start[p + 1] - first, do_solve, gw, fw, sh, tot);
 }
 
 template <typename S>
@@ -137,13 +141,11 @@ __global__ void __launch_bounds__(kBlock)
       const int64_t idx = base + k;
       if (idx < L.n) information_pixel(L, m, map, idx, acc);
     }
-    block_partial(acc, partials + int64_t(t) * kSlots, sh);
+    block_partial(acc, partials + int64_t(t) * kSlots, sh, true);
     if (arrive_last(&tickets[p], ntiles - 1, &s_last)) {
-      sum_partials(partials + int64_t(first) * kSlots, ntiles, sh, tot);
-      if (threadIdx.x == 0) {
-        tickets[p] = 0;
-        store_information(st, tot);
-      }
+      sum_partials<kBlock>(partials + int64_t(first) * kSlots, ntiles, sh, tot);
+      if (threadIdx.x == 0) tickets[p] = 0;
+      store_information(st, tot);
     }
     __syncthreads();
   }
@@ -202,12 +204,16 @@ int grid_tiles(const stb_icp_params* P, int tiles, int dflt) {
 }
 
 template <typename S, bool GEOM, bool FEAT>
-void launch_batch_jacobian(const void* plan, stb_icp_state* states,
-                           const Scratch& sc, int grid, const TermWeights& tw,
-                           int do_solve, int fixed, hipStream_t s) {
+void launch_batch_jacobian(const void* plan, int num_pairs,
+                           stb_icp_state* states, const Scratch& sc, int grid,
+                           const TermWeights& tw, int do_solve, int fixed,
+                           hipStream_t s) {
   hipLaunchKernelGGL((batch_jacobian_kernel<S, GEOM, FEAT>), dim3(grid),
                      dim3(kBlock), 0, s, plan, states, sc.map0, sc.map1,
-                     sc.partials, sc.tickets, tw, do_solve, fixed);
+                     sc.partials, tw, fixed);
+  hipLaunchKernelGGL(batch_solve_kernel, dim3(num_pairs), dim3(kSolveBlock), 0,
+                     s, plan, states, sc.partials, tw.gw, tw.fw, do_solve,
+                     fixed);
 }
 
 enum BatchPhase { kIterate, kLinearize, kInformation };
@@ -243,14 +249,14 @@ void launch_batch(const void* plan, const stb_icp_batch_sizes& z,
     }
     const int solve = phase == kIterate ? 1 : 0;
     if (P.use_geom && P.use_feat)
-      launch_batch_jacobian<S, true, true>(plan, states, sc, gj, tw, solve,
-                                           fixed, s);
+      launch_batch_jacobian<S, true, true>(plan, z.num_pairs, states, sc, gj,
+                                           tw, solve, fixed, s);
     else if (P.use_geom)
-      launch_batch_jacobian<S, true, false>(plan, states, sc, gj, tw, solve,
-                                            fixed, s);
+      launch_batch_jacobian<S, true, false>(plan, z.num_pairs, states, sc, gj,
+                                            tw, solve, fixed, s);
     else
-      launch_batch_jacobian<S, false, true>(plan, states, sc, gj, tw, solve,
-                                            fixed, s);
+      launch_batch_jacobian<S, false, true>(plan, z.num_pairs, states, sc, gj,
+                                            tw, solve, fixed, s);
   }
 }
 

@@ -1,6 +1,10 @@
-// Fused single-pair Gauss-Newton path (ICPOdometry): per iteration one
-// correspondence kernel and one Jacobian/reduction kernel whose last block
-// solves the 6x6 system and updates the pose on the device.
+// Fused single-pair Gauss-Newton path (ICPOdometry): per iteration a
+// correspondence kernel, a Jacobian kernel that leaves one FP64 partial per
+// block, and a one-block kernel that sums the partials in a fixed order,
+// solves the 6x6 system and updates the pose on the device. (A last-block
+// ticket inside the Jacobian launch would cost every block a device-scope
+// fence and an atomic on one shared counter, about 20 ns per block in all --
+// 10 us for 512 blocks, several times the cost of the third launch.)
 #include "gn_device.hpp"
 
 namespace stb {
@@ -37,16 +41,16 @@ __global__ void __launch_bounds__(kBlock)
     corresp_pixel(L, m, nm, idx, map, dthr2, cstar);
 }
 
+// Per-pixel Jacobians of one iteration; every block leaves one FP64 partial
+// for solve_fused_kernel (the next launch), so no block waits on another.
 template <typename S, bool GEOM, bool FEAT>
 __global__ void __launch_bounds__(kBlock)
     jacobian_fused_kernel(stb_icp_level Ld, stb_icp_state* __restrict__ st,
                           unsigned long long* __restrict__ map0,
                           unsigned long long* __restrict__ map1,
                           int64_t map_len, double* __restrict__ partials,
-                          TermWeights tw, int do_solve, int fixed_map) {
+                          TermWeights tw, int fixed_map) {
   __shared__ double sh[kShared];
-  __shared__ double tot[kSlots];
-  __shared__ int s_last;
   if (!fixed_map && st->stopped) return;
   const unsigned long long* __restrict__ map =
       pick_map(st, map0, map1, fixed_map);
@@ -65,16 +69,35 @@ __global__ void __launch_bounds__(kBlock)
   for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
   for (int64_t idx = gtid; idx < L.n; idx += gsize)
     jacobian_pixel<S, GEOM, FEAT>(L, m, map, idx, tw, acc);
-  block_partial(acc, partials + int64_t(blockIdx.x) * kSlots, sh);
-  if (!arrive_last(&st->ticket, int(gridDim.x) - 1, &s_last)) return;
-  sum_partials(partials, int(gridDim.x), sh, tot);
-  if (threadIdx.x == 0) {
-    st->ticket = 0;
-    if (do_solve)
-      gn_finalize(st, tot, tw.gw, tw.fw, sh);
-    else
-      store_linearization(st, tot);
+  block_partial(acc, partials + int64_t(blockIdx.x) * kSlots, sh, false);
+}
+
+// The fixed-order sum of the Jacobian launch's `count` partials, then the
+// pose update (do_solve) or the stored linearisation. kSolveGroups blocks
+// each sum a contiguous range of partials into a group partial; the last to
+// arrive adds the groups in order and finishes -- a handful of fences and
+// tickets instead of one per Jacobian block, and many CUs pulling partials.
+__global__ void __launch_bounds__(kBlock)
+    solve_fused_kernel(stb_icp_state* __restrict__ st,
+                       const double* __restrict__ partials, int count,
+                       double* __restrict__ groups, double gw, double fw,
+                       int do_solve, int fixed_map) {
+  __shared__ double sh[kShared];
+  __shared__ double tot[kSlots];
+  __shared__ int s_last;
+  if (!fixed_map && st->stopped) return;
+  const int G = int(gridDim.x), g = int(blockIdx.x);
+  const int lo = int(int64_t(count) * g / G);
+  const int hi = int(int64_t(count) * (g + 1) / G);
+  sum_partials<kBlock>(partials + int64_t(lo) * kSlots, hi - lo, sh, tot);
+  if (threadIdx.x < kSlots) {
+    groups[g * kSlots + threadIdx.x] = tot[threadIdx.x];
+    __threadfence();
   }
+  __syncthreads();
+  if (!arrive_last(&st->ticket, G - 1, &s_last)) return;
+  if (threadIdx.x == 0) st->ticket = 0;
+  finish_partials<kBlock>(st, groups, G, do_solve, gw, fw, sh, tot);
 }
 
 template <typename S>
@@ -97,13 +120,11 @@ __global__ void __launch_bounds__(kBlock)
   for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < L.n;
        idx += int64_t(gridDim.x) * blockDim.x)
     information_pixel(L, m, map, idx, acc);
-  block_partial(acc, partials + int64_t(blockIdx.x) * kSlots, sh);
+  block_partial(acc, partials + int64_t(blockIdx.x) * kSlots, sh, true);
   if (!arrive_last(&st->ticket, int(gridDim.x) - 1, &s_last)) return;
-  sum_partials(partials, int(gridDim.x), sh, tot);
-  if (threadIdx.x == 0) {
-    st->ticket = 0;
-    store_information(st, tot);
-  }
+  sum_partials<kBlock>(partials, int(gridDim.x), sh, tot);
+  if (threadIdx.x == 0) st->ticket = 0;
+  store_information(st, tot);
 }
 
 __global__ void icp_begin_kernel(stb_icp_state* st,
@@ -138,19 +159,22 @@ __global__ void __launch_bounds__(kBlock)
 
 constexpr int kDefaultUpdateBlocks = 512;
 constexpr int kMaxUpdateBlocks = 2048;
+constexpr int kSolveGroups = 16;
 constexpr int kDefaultCorrespBlocks = 4096;
 
 struct Workspace {
   stb_icp_state* st;
   unsigned long long* map[2];
   double* partials;
+  double* groups;
   int64_t map_len;
 };
 
 size_t ws_bytes(int32_t H, int32_t W) {
   const size_t map_len = size_t(H) * size_t(W);
   return align256(sizeof(stb_icp_state)) + 2 * align256(map_len * 8) +
-         align256(size_t(kMaxUpdateBlocks) * kSlots * sizeof(double));
+         align256(size_t(kMaxUpdateBlocks) * kSlots * sizeof(double)) +
+         align256(size_t(kSolveGroups) * kSlots * sizeof(double));
 }
 
 Workspace carve(void* base, int32_t H, int32_t W) {
@@ -164,6 +188,8 @@ Workspace carve(void* base, int32_t H, int32_t W) {
   w.map[1] = reinterpret_cast<unsigned long long*>(p);
   p += align256(size_t(w.map_len) * 8);
   w.partials = reinterpret_cast<double*>(p);
+  p += align256(size_t(kMaxUpdateBlocks) * kSlots * sizeof(double));
+  w.groups = reinterpret_cast<double*>(p);
   return w;
 }
 
@@ -206,10 +232,14 @@ template <typename S, bool GEOM, bool FEAT>
 void launch_jacobian(const stb_icp_level& L, const stb_icp_params& P,
                      const Workspace& w, const TermWeights& tw, int do_solve,
                      int fixed_map, hipStream_t stream) {
-  hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT>),
-                     dim3(update_grid(P, L.n_src)), dim3(kBlock), 0, stream, L,
-                     w.st, w.map[0], w.map[1], w.map_len, w.partials, tw,
-                     do_solve, fixed_map);
+  const int grid = update_grid(P, L.n_src);
+  hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT>), dim3(grid),
+                     dim3(kBlock), 0, stream, L, w.st, w.map[0], w.map[1],
+                     w.map_len, w.partials, tw, fixed_map);
+  const int groups = grid < kSolveGroups ? grid : kSolveGroups;
+  hipLaunchKernelGGL(solve_fused_kernel, dim3(groups), dim3(kBlock), 0, stream,
+                     w.st, w.partials, grid, w.groups, tw.gw, tw.fw, do_solve,
+                     fixed_map);
 }
 
 template <typename S>

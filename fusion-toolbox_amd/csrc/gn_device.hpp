@@ -5,8 +5,9 @@
 //     descriptor (any map storage type, FP32 math);
 //   * the slot reduction: 21 unique JtJ + 6 Jr + r2/count bookkeeping per
 //     lane in FP32, recursive-halving across the wavefront (32 exchanges for
-//     32 slots), FP64 across waves through LDS, and a fixed-order FP64 sum of
-//     per-block partials by the last block to arrive;
+//     32 slots), FP64 across waves through LDS, one FP64 partial per block,
+//     and a fixed-order FP64 sum of the partials by one block of a following
+//     launch (or, for one-off passes, by the last block to arrive);
 //   * the per-pair state update: FP64 Cholesky solve and exp_rt_to_matrix.
 #pragma once
 
@@ -18,6 +19,9 @@ constexpr int kSlots = 32;  // 21 JtJ + 6 Jr + r2_geom, r2_feat, n_geom, n_feat,
 constexpr int kWaves = kBlock / 64;
 constexpr int kGroups = kBlock / kSlots;  // slot groups of the final sum
 constexpr int kShared = (kGroups > kWaves ? kGroups : kWaves) * kSlots;
+// Block size of the launches that sum per-block partials and update poses:
+// 32 slot groups share the partial loads.
+constexpr int kSolveBlock = 1024;
 
 // J scales and residual signs of the combined system of
 // slamtb/registration/icp.py:221-234: JtJ = fw^2 JtJ_f + gw^2 JtJ_g and
@@ -247,10 +251,13 @@ __device__ inline int bitrev5(int l) {
          ((l & 16) >> 4);
 }
 
-// Reduces the block's slots and stores them in FP64 at `out` (32 doubles),
-// made visible device-wide. `sh` is LDS of kShared doubles.
+// Reduces the block's slots and stores them in FP64 at `out` (32 doubles).
+// With `publish` the stores are fenced device-wide, for a consumer in the
+// same launch (arrive_last); a consumer in a later launch needs no fence, and
+// each fence costs a few nanoseconds per block at the memory side.
+// `sh` is LDS of kShared doubles.
 __device__ inline void block_partial(float (&acc)[kSlots], double* out,
-                                     double* sh) {
+                                     double* sh, bool publish) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   wave_reduce_slots(acc, lane);
@@ -261,7 +268,7 @@ __device__ inline void block_partial(float (&acc)[kSlots], double* out,
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) s += sh[w * kSlots + threadIdx.x];
     out[threadIdx.x] = s;
-    __threadfence();
+    if (publish) __threadfence();
   }
   __syncthreads();
 }
@@ -277,23 +284,39 @@ __device__ inline bool arrive_last(int32_t* counter, int expected,
   return last;
 }
 
-// Fixed-order FP64 sum of `count` partials (32 doubles each) into tot[32]:
-// group g of kGroups adds partials g, g + kGroups, ..., then the groups are
-// added in order -- the same order whichever block performs it.
+// Fixed-order FP64 sum of `count` partials (32 doubles each) into tot[32] by
+// a block of BLOCK threads: group g of BLOCK / 32 adds partials g,
+// g + BLOCK / 32, ..., then the groups are added in order -- the same order
+// whichever block performs it. The partials were written by blocks on every
+// XCD, so each load misses this XCD's L2: kSumBatch independent loads are
+// issued before their in-order adds, paying that latency once per batch
+// instead of once per partial. `sh` is LDS of BLOCK doubles.
+constexpr int kSumBatch = 16;
+
+template <int BLOCK>
 __device__ inline void sum_partials(const double* partials, int count,
                                     double* sh, double* tot) {
+  constexpr int kG = BLOCK / kSlots;
   const int slot = threadIdx.x & (kSlots - 1);
   const int grp = threadIdx.x / kSlots;
   double s = 0.0;
-  for (int b = grp; b < count; b += kGroups)
-    s += partials[int64_t(b) * kSlots + slot];
+  for (int b0 = grp; b0 < count; b0 += kSumBatch * kG) {
+    double v[kSumBatch];
+#pragma unroll
+    for (int u = 0; u < kSumBatch; ++u) {
+      const int b = b0 + u * kG;
+      v[u] = b < count ? partials[int64_t(b) * kSlots + slot] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSumBatch; ++u) s += v[u];
+  }
   __syncthreads();
   sh[grp * kSlots + slot] = s;
   __syncthreads();
   if (threadIdx.x < kSlots) {
     double t = 0.0;
 #pragma unroll
-    for (int g = 0; g < kGroups; ++g) t += sh[g * kSlots + threadIdx.x];
+    for (int g = 0; g < kG; ++g) t += sh[g * kSlots + threadIdx.x];
     tot[threadIdx.x] = t;
   }
   __syncthreads();
@@ -328,7 +351,9 @@ __device__ inline void exp_rt_f64(const double x[6], double U[12]) {
     ay = wy / th;
     az = wz / th;
   }
-  const double s = sin(th), c = cos(th), k = 1.0 - c;
+  double s, c;
+  sincos(th, &s, &c);
+  const double k = 1.0 - c;
   U[0] = k * ax * ax + c;
   U[1] = k * ax * ay - s * az;
   U[2] = k * ax * az + s * ay;
@@ -343,114 +368,169 @@ __device__ inline void exp_rt_f64(const double x[6], double U[12]) {
   U[11] = x[2];
 }
 
+// Index of JtJ[a][c] (a <= c) among the 21 packed slots (upper triangle,
+// row-major), and of L[i][j] (j <= i) in a packed lower triangle.
+__host__ __device__ constexpr int tri_up(int a, int c) {
+  return a * (11 - a) / 2 + c;
+}
+__host__ __device__ constexpr int tri_lo(int i, int j) {
+  return i * (i + 1) / 2 + j;
+}
+__device__ inline double sym6(const double* tot, int a, int c) {
+  return a <= c ? tot[tri_up(a, c)] : tot[tri_up(c, a)];
+}
+
+// The tails below are block-cooperative: every thread of the block that
+// finished the reduction calls them (tot and w in LDS). Stores to the state
+// are spread over threads and every value the serial part needs is read into
+// LDS or registers first, so the tail pays one memory round trip, not one
+// per element.
+
 // Stores the reduced system of one linearisation in the state.
-__device__ inline void store_linearization(stb_icp_state* st, const double* tot) {
-  int k = 0;
-#pragma unroll 1
-  for (int a = 0; a < 6; ++a)
-#pragma unroll 1
-    for (int c = a; c < 6; ++c) {
-      st->last_jtj[a * 6 + c] = tot[k];
-      st->last_jtj[c * 6 + a] = tot[k];
-      ++k;
-    }
-#pragma unroll 1
-  for (int a = 0; a < 6; ++a) st->last_jr[a] = tot[21 + a];
-  st->last_geom_r2 = tot[27];
-  st->last_feat_r2 = tot[28];
-  st->last_geom_count = tot[29];
-  st->last_feat_count = tot[30];
+__device__ inline void store_linearization(stb_icp_state* st,
+                                           const double* tot) {
+  const int t = threadIdx.x;
+  if (t < 36)
+    st->last_jtj[t] = sym6(tot, t / 6, t % 6);
+  else if (t < 42)
+    st->last_jr[t - 36] = tot[21 + t - 36];
+  else if (t == 42)
+    st->last_geom_r2 = tot[27];
+  else if (t == 43)
+    st->last_feat_r2 = tot[28];
+  else if (t == 44)
+    st->last_geom_count = tot[29];
+  else if (t == 45)
+    st->last_feat_count = tot[30];
 }
 
 // One Gauss-Newton update from the reduced totals (slamtb/registration/icp.py:
 // 236-256): FP64 Cholesky of JtJ (a non-positive pivot is the reference's
-// `except: break`), solve JtJ x = Jr, T <- exp_rt(x) T. Runs in one thread
-// on matrices kept in LDS (`w` holds 36 + 36 + 6 + 12 doubles) with its
-// loops left rolled, so that the kernel's register budget is set by the
-// per-pixel loop rather than by this tail. Flips map_sel on success.
+// `except: break` and stops the loop), solve JtJ x = Jr, T <- exp_rt(x) T.
+// The factorisation, solve and pose product run in one thread on registers
+// (fully unrolled over packed triangles): the serial chain is FP64
+// arithmetic only. It runs in a launch of its own, so its register count
+// does not bound the occupancy of the per-pixel kernels. `w` is LDS of at
+// least 25 doubles whose first 12 hold rows 0..2 of the current pose;
+// thread 0 passes the state's iteration count and map_sel, read early by the
+// caller so that their latency hides behind the partial sums. Flips map_sel
+// on success.
 __device__ inline void gn_finalize(stb_icp_state* st, const double* tot,
-                                   double gw, double fw, double* w) {
-  double* A = w;
-  double* L = w + 36;
-  double* x = w + 72;
-  double* U = w + 78;
+                                   double gw, double fw, double* w, int iters,
+                                   int sel) {
+  const int t = threadIdx.x;
   store_linearization(st, tot);
-#pragma unroll 1
-  for (int i = 0; i < 36; ++i) {
-    A[i] = st->last_jtj[i];
-    L[i] = 0.0;
-  }
-#pragma unroll 1
-  for (int j = 0; j < 6; ++j) {
-    double s = A[j * 6 + j];
-#pragma unroll 1
-    for (int q = 0; q < j; ++q) s -= L[j * 6 + q] * L[j * 6 + q];
-    if (!(s > 0.0)) {
+  __syncthreads();
+  if (t == 0) {
+    double A[21], b[6], T[12];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) A[k] = tot[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) b[k] = tot[21 + k];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) T[k] = w[k];
+    double L[21], rinv[6];
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double d2 = A[tri_up(j, j)];
+#pragma unroll
+      for (int q = 0; q < j; ++q) d2 -= L[tri_lo(j, q)] * L[tri_lo(j, q)];
+      ok = ok && (d2 > 0.0);
+      const double d = sqrt(d2);
+      L[tri_lo(j, j)] = d;
+      rinv[j] = 1.0 / d;
+#pragma unroll
+      for (int i = j + 1; i < 6; ++i) {
+        double v = A[tri_up(j, i)];
+#pragma unroll
+        for (int q = 0; q < j; ++q) v -= L[tri_lo(i, q)] * L[tri_lo(j, q)];
+        L[tri_lo(i, j)] = v * rinv[j];
+      }
+    }
+    if (ok) {
+      double y[6], x[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {  // L y = b
+        double v = b[i];
+#pragma unroll
+        for (int q = 0; q < i; ++q) v -= L[tri_lo(i, q)] * y[q];
+        y[i] = v * rinv[i];
+      }
+#pragma unroll
+      for (int i = 5; i >= 0; --i) {  // L^T x = y
+        double v = y[i];
+#pragma unroll
+        for (int q = i + 1; q < 6; ++q) v -= L[tri_lo(q, i)] * x[q];
+        x[i] = v * rinv[i];
+      }
+      double U[12];
+      exp_rt_f64(x, U);
+      Rigid<float> r;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          double v = U[i * 4 + 0] * T[j] + U[i * 4 + 1] * T[4 + j] +
+                     U[i * 4 + 2] * T[8 + j];
+          if (j == 3) v += U[i * 4 + 3];
+          w[12 + i * 4 + j] = v;
+          r.m[i * 4 + j] = float(v);
+        }
+      make_normal_matrix(r);
+#pragma unroll
+      for (int i = 0; i < 12; ++i) st->rt[i] = r.m[i];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) st->nm[i] = r.nm[i];
+      const double count = fmax(tot[29], tot[30]);
+      st->best_residual = (fw * tot[28] + gw * tot[27]) / count;
+      st->best_match_count = count;
+      st->iterations = iters + 1;
+      st->map_sel = sel ^ 1;
+    } else {
       st->stopped = 1;
-      return;
     }
-    const double d = sqrt(s);
-    L[j * 6 + j] = d;
-#pragma unroll 1
-    for (int i = j + 1; i < 6; ++i) {
-      double t = A[i * 6 + j];
-#pragma unroll 1
-      for (int q = 0; q < j; ++q) t -= L[i * 6 + q] * L[j * 6 + q];
-      L[i * 6 + j] = t / d;
+    w[24] = ok ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  if (w[24] != 0.0) {
+    if (t < 12) {
+      st->transform[t] = w[12 + t];
+      st->best_transform[t] = w[12 + t];
+    } else if (t >= 64 && t < 100) {
+      st->best_jtj[t - 64] = sym6(tot, (t - 64) / 6, (t - 64) % 6);
     }
   }
-#pragma unroll 1
-  for (int i = 0; i < 6; ++i) {  // L y = b (y kept in x)
-    double t = tot[21 + i];
-#pragma unroll 1
-    for (int q = 0; q < i; ++q) t -= L[i * 6 + q] * x[q];
-    x[i] = t / L[i * 6 + i];
-  }
-#pragma unroll 1
-  for (int i = 5; i >= 0; --i) {  // L^T x = y
-    double t = x[i];
-#pragma unroll 1
-    for (int q = i + 1; q < 6; ++q) t -= L[q * 6 + i] * x[q];
-    x[i] = t / L[i * 6 + i];
-  }
-  exp_rt_f64(x, U);
-  // T <- U T (rows 0..2; the last row stays [0 0 0 1]); L is free again and
-  // stages the new rows.
-#pragma unroll 1
-  for (int i = 0; i < 3; ++i)
-#pragma unroll 1
-    for (int j = 0; j < 4; ++j) {
-      double t = U[i * 4 + 0] * st->transform[0 * 4 + j] +
-                 U[i * 4 + 1] * st->transform[1 * 4 + j] +
-                 U[i * 4 + 2] * st->transform[2 * 4 + j];
-      if (j == 3) t += U[i * 4 + 3];
-      L[i * 4 + j] = t;
-    }
-#pragma unroll 1
-  for (int i = 0; i < 12; ++i) {
-    st->transform[i] = L[i];
-    st->best_transform[i] = L[i];
-  }
-#pragma unroll 1
-  for (int i = 0; i < 36; ++i) st->best_jtj[i] = A[i];
-  const double count = fmax(tot[29], tot[30]);
-  st->best_residual = (fw * tot[28] + gw * tot[27]) / count;
-  st->best_match_count = count;
-  st->iterations += 1;
-  st->map_sel ^= 1;
-  set_pose_floats(st);
 }
 
 __device__ inline void store_information(stb_icp_state* st, const double* tot) {
-  int k = 0;
-#pragma unroll 1
-  for (int a = 0; a < 6; ++a)
-#pragma unroll 1
-    for (int c = a; c < 6; ++c) {
-      st->information[a * 6 + c] = tot[k];
-      st->information[c * 6 + a] = tot[k];
-      ++k;
-    }
+  const int t = threadIdx.x;
+  if (t < 36) st->information[t] = sym6(tot, t / 6, t % 6);
+}
+
+// Sums `count` partials and applies them to the state: a Gauss-Newton update
+// (do_solve) or only the stored linearisation. Called by every thread of one
+// block of BLOCK threads; `sh` is LDS of BLOCK doubles, `tot` of kSlots.
+template <int BLOCK>
+__device__ inline void finish_partials(stb_icp_state* st, const double* partials,
+                                       int count, int do_solve, double gw,
+                                       double fw, double* sh, double* tot) {
+  const int t = threadIdx.x;
+  // The state loads of the update issue before the partial loads.
+  double row = 0.0;
+  int iters = 0, sel = 0;
+  if (do_solve && t < 12) row = st->transform[t];
+  if (do_solve && t == 0) {
+    iters = st->iterations;
+    sel = st->map_sel;
+  }
+  sum_partials<BLOCK>(partials, count, sh, tot);
+  if (do_solve) {
+    if (t < 12) sh[t] = row;
+    gn_finalize(st, tot, gw, fw, sh, iters, sel);
+  } else {
+    store_linearization(st, tot);
+  }
 }
 
 // Resets one state to pose T0 (row-major 4x4, NULL = identity) or, with

@@ -127,8 +127,13 @@ def test_multiscale_registration_matches_oracle():
                                  (0.5, ICPOdometry(5, **kw)),
                                  (0.5, ICPOdometry(3, **kw))])
     feats = [g(pr["src_feats"]), g(pr["tgt_feats"])]
+    # The oracle gets the levels of clones taken before the registration:
+    # a frame whose normals were computed once carries them into later
+    # pyramids instead of re-estimating them per level (frame.py:404-408),
+    # so a second pyramid of the same frames is a different input.
+    clones = [fp.clone() for fp in fps]
     res = ms.estimate_frame(fps[0], fps[1], feats[0], feats[1], device=DEV)
-    levels = ms.levels(fps[0], fps[1], feats[0], feats[1])
+    levels = ms.levels(clones[0], clones[1], feats[0], feats[1])
     T = np.eye(4)
     for est, L in levels:
         kk = np.array([[L.fx, 0, L.cx], [0, L.fy, L.cy], [0, 0, 1]], np.float32)
