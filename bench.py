@@ -1,32 +1,44 @@
 #!/usr/bin/env python3
 """Benchmark of the ICP Gauss-Newton step on one MI355X.
 
-Workload (the 640x480 multiscale geometric + feature configuration):
+Headline workload (640x480 multiscale, geometric + 3-channel features):
 a seeded synthetic scene (6 planes, 4 spheres) rendered to 640x480 float32
 depth with fx = fy = 525, cx = 319.5, cy = 239.5 and noise 0.0012 z^2, a
 3-channel float32 feature map (albedo + sinusoidal texture), the target moved
 by a random rigid motion (<= 5 deg, <= 5 cm). Preprocessing on the GPU:
 bilateral filter (sigma_s 4.5 px, sigma_r 0.03 m, width 6), erosion,
 back-projection, central-difference normals, 3-level pyramid (640x480,
-320x240, 160x120) with feature blur + 2x2 mean. ICP: geometric weight 1,
-feature weight 1 (geometric 10), distance 0.1 m, normal angle 20 deg, feature residual
-threshold 2.75, 10/5/3 iterations. The 1:0.1 geometric:feature ratio is given as
-geom_weight=10, feat_weight=1, the form of the reference preset
-(slamtb/registration/preset.py:10); with geom 1 : feat 0.1 the reference
+320x240, 160x120) with feature blur + 2x2 mean. ICP: distance 0.1 m, normal
+angle 20 deg, feature residual threshold 2.75, 10/5/3 iterations, weights
+geometric 10 : feature 1 -- the 1:0.1 ratio in the form of the reference
+preset (slamtb/registration/preset.py:10); written as 1 : 0.1 the reference
 weighting (JtJ by w^2, Jr by w) over-steps the feature term and diverges.
 
-A step is one Gauss-Newton iteration at 640x480 (correspondence kernel +
-fused Jacobian/reduction/solve kernel) on inputs resident in HBM; K steps are
-captured in a hipGraph and timed with one replay. Also reported: the full
-3-level registration per pair, pose error against the oracle fed the same
-inputs (FP32 per-pixel arithmetic, FP64 sums and solve) and against the
-ground truth, the dominant kernel's roofline and the CPU baseline (the
-oracle's OpenMP port of the reference CPU path on this host's cores).
+A step is one Gauss-Newton iteration at 640x480 (correspondence launch,
+Jacobian launch, reduction/solve launch) on inputs resident in HBM; K steps
+are captured in one hipGraph and timed with one replay between
+synchronisations. Also reported:
+  * per-launch durations of the correspondence and Jacobian kernels (HIP
+    events around a captured graph of back-to-back launches of that kernel
+    alone) and the dominant kernel's roofline against 8 TB/s, with the HBM
+    bytes that rocprofv3 counters measured for it (profiles/) and a
+    device-to-device copy ceiling measured in the same process;
+  * the full 3-level registration per pair, its pose error against the
+    oracle fed the same level inputs -- FP32 per-pixel arithmetic with FP64
+    sums (decision-matched) and all-FP64 -- and against the ground truth;
+  * 64 pairs of uneven resolution in one batched hipGraph against the same
+    pairs one after another through the single-pair path;
+  * 1280x720 with 32-channel descriptors in BF16 storage against FP32;
+  * 5 unrolled differentiable GN steps on 8 pairs at 640x480, forward and
+    adjoint backward;
+  * the CPU baseline: the oracle's OpenMP port of the reference CPU path on
+    this host, at all threads and at one thread.
 """
 import argparse
 import json
 import math
-import os
+import platform
+import statistics
 import sys
 import time
 from pathlib import Path
@@ -36,6 +48,9 @@ sys.path.insert(0, str(ROOT / "fusion-toolbox_amd"))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "ICP GN iters/s @640x480; batched pairs/s vs plain; %HBM peak; pose err"
+TRAFFIC_FILE = ROOT / "profiles" / "jacobian_fused_kernel_640x480_traffic.json"
+KW = dict(geom_weight=10.0, feat_weight=1.0, distance_threshold=0.1,
+          normals_angle_thresh=math.radians(20), feat_residual_thresh=2.75)
 
 
 def parse():
@@ -52,7 +67,14 @@ def parse():
                     help="pairs of the batched workload (0 disables it)")
     ap.add_argument("--no-bf16", action="store_true",
                     help="skip the 1280x720 BF16-storage workload")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-diff", action="store_true",
+                    help="skip the differentiable 8 x 640x480 workload")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="batched workload: default tile and grid instead of "
+                         "the autotuned ones")
+    ap.add_argument("--tune", action="store_true",
+                    help="640x480 steps: autotuned grids instead of the "
+                         "library defaults")
     ap.add_argument("--profile", action="store_true",
                     help="only the 640x480 steps and the per-kernel timing "
                          "(for profiler runs: every launch is the same kernel "
@@ -63,9 +85,13 @@ def parse():
     return ap.parse_args()
 
 
-def copy_probe(mib):
-    """A plain device-to-device copy of known size: what the HBM byte
-    counters read for it scales the counters of the ICP kernels."""
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def copy_rate(mib, reps=10):
+    """Device-to-device copy of a known size: (bytes per copy, GB/s read +
+    written)."""
     import torch
     n = mib << 20
     a = torch.ones(n, dtype=torch.uint8, device="cuda:0")
@@ -73,118 +99,32 @@ def copy_probe(mib):
     b.copy_(a)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(10):
+    for _ in range(reps):
         b.copy_(a)
     torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / 10
+    dt = (time.perf_counter() - t0) / reps
+    return n, 2 * n / dt / 1e9
+
+
+def copy_probe(mib):
+    """What the HBM byte counters read for this copy scales the counters of
+    the ICP kernels."""
+    n, gbs = copy_rate(mib)
     print(json.dumps({"copy_probe_bytes": n, "read_bytes": n, "write_bytes": n,
-                      "gbs": 2 * n / dt / 1e9}), flush=True)
-
-
-def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+                      "gbs": gbs}), flush=True)
 
 
 def prepare(seed, dev):
     import numpy as np
-    import torch
     from slamtb import synthetic
-    from slamtb.camera import KCamera
-    from slamtb.frame import Frame, FrameInfo, FramePointCloud
-    from slamtb.processing import bilateral_depth_filter
     from slamtb.registration import ICPOdometry, MultiscaleRegistration
 
-    k = synthetic.config_kcam(2)
+    k = synthetic.intrinsics(640, 480).astype(np.float32)
     pair = synthetic.make_pair(seed, 640, 480, kcam=k, channels=3)
-    kcam = KCamera(torch.from_numpy(k))
-    fpcls = []
-    for side in ("src", "tgt"):
-        depth = torch.from_numpy(pair[f"{side}_depth"]).to(dev)
-        depth = bilateral_depth_filter(depth, depth > 0, None, 6, 0.03, 4.5, 1.0)
-        frame = Frame(FrameInfo(kcam, 1.0, 0.0), depth_image=depth)
-        fpcls.append(FramePointCloud.from_frame(frame, device=dev))
-    ang = math.radians(20)
-    kw = dict(geom_weight=10.0, feat_weight=1.0, distance_threshold=0.1,
-              normals_angle_thresh=ang, feat_residual_thresh=2.75)
-    ms = MultiscaleRegistration([(1.0, ICPOdometry(10, **kw)),
-                                 (0.5, ICPOdometry(5, **kw)),
-                                 (0.5, ICPOdometry(3, **kw))])
-    feats = [torch.from_numpy(pair[f"{s}_feats"]).to(dev) for s in ("src", "tgt")]
-    levels = ms.levels(fpcls[0], fpcls[1], feats[0], feats[1])
-    return pair, ms, levels
-
-
-def time_iteration_kernels(run, level, params, reps):
-    """Average duration of each of the two kernels of an iteration, from
-    event pairs recorded around each launch on the stream they run on."""
-    import torch
-    s = torch.cuda.current_stream()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)]
-          for _ in range(reps)]
-    for i in range(reps):
-        ev[i][0].record(s)
-        run.correspond(level, params)
-        ev[i][1].record(s)
-        run.update(level, params)
-        ev[i][2].record(s)
-    torch.cuda.synchronize()
-    a = sorted(e[0].elapsed_time(e[1]) for e in ev)
-    b = sorted(e[1].elapsed_time(e[2]) for e in ev)
-    return (sum(a) / reps, a[reps // 2]), (sum(b) / reps, b[reps // 2])
-
-
-def oracle_levels(levels, T0=None):
-    """Run the oracle GN over the same (GPU-prepared) level inputs."""
-    import numpy as np
-    sys.path.insert(0, str(ROOT))
-    from oracle import oracle as orc
-    T = np.eye(4) if T0 is None else T0
-    res = None
-    for est, L in levels:
-        k = np.array([[L.fx, 0, L.cx], [0, L.fy, L.cy], [0, 0, 1]], np.float32)
-        c = lambda t: t.detach().cpu().numpy()
-        res = orc.icp_estimate(
-            k, c(L.src_points), c(L.src_normals), c(L.src_mask),
-            c(L.tgt_points), c(L.tgt_normals), c(L.tgt_mask), est.num_iters,
-            src_feats=c(L.src_feats) if L.has_features else None,
-            tgt_feats=c(L.tgt_feats) if L.has_features else None, transform=T,
-            geom_weight=est.geom_weight, feat_weight=est.feat_weight,
-            distance_threshold=est.distance_threshold,
-            normals_angle_thresh=est.normals_angle_thresh,
-            feat_residual_thresh=est.feat_residual_thresh)
-        T = res["transform"] if res["transform"] is not None else np.eye(4)
-    return res
-
-
-def cpu_baseline(level, est, seconds):
-    """Reference-structured CPU path (OpenMP per-pixel loops, mutex scatter,
-    materialised partials, FP32 sums, FP64 solve) timed on this host."""
-    import numpy as np
-    sys.path.insert(0, str(ROOT))
-    from oracle import oracle as orc
-    c = lambda t: t.detach().cpu().numpy()
-    k = np.array([[level.fx, 0, level.cx], [0, level.fy, level.cy], [0, 0, 1]],
-                 np.float32)
-    args = (k, c(level.src_points), c(level.src_normals), c(level.src_mask),
-            c(level.tgt_points), c(level.tgt_normals), c(level.tgt_mask))
-    kw = dict(src_feats=c(level.src_feats), tgt_feats=c(level.tgt_feats),
-              geom_weight=est.geom_weight, feat_weight=est.feat_weight,
-              distance_threshold=est.distance_threshold,
-              normals_angle_thresh=est.normals_angle_thresh,
-              feat_residual_thresh=est.feat_residual_thresh,
-              sum_dtype=np.float32, parallel=True, with_information=False)
-    orc.icp_estimate(*args, 1, **kw)  # warm-up
-    iters, t0 = 0, time.perf_counter()
-    T = np.eye(4)
-    while True:
-        r = orc.icp_estimate(*args, 1, transform=T, **kw)
-        if r["transform"] is not None:
-            T = r["transform"]
-        iters += 1
-        el = time.perf_counter() - t0
-        if el >= seconds and iters >= 3:
-            break
-    return iters / el, iters, el, orc.num_threads()
+    ms = MultiscaleRegistration([(1.0, ICPOdometry(10, **KW)),
+                                 (0.5, ICPOdometry(5, **KW)),
+                                 (0.5, ICPOdometry(3, **KW))])
+    return pair, ms, gpu_pair_levels(ms, pair, dev)
 
 
 def gpu_pair_levels(ms, pair, dev, storage=None):
@@ -205,12 +145,13 @@ def gpu_pair_levels(ms, pair, dev, storage=None):
 
 
 def capture(fn):
+    """hipGraph of fn() (run once uncaptured first)."""
     import torch
     g = torch.cuda.CUDAGraph()
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
-        fn()  # run once uncaptured (warms caches, allocations)
+        fn()
         with torch.cuda.graph(g, stream=side):
             fn()
     torch.cuda.current_stream().wait_stream(side)
@@ -219,6 +160,7 @@ def capture(fn):
 
 
 def time_graph(g, reps):
+    """Mean wall time of one replay (host clock around synchronisations)."""
     import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -228,13 +170,130 @@ def time_graph(g, reps):
     return (time.perf_counter() - t0) / reps
 
 
+def launch_ms(fn, reps=100):
+    """Average duration of one launch of fn: HIP events on the replaying
+    stream around a hipGraph of `reps` back-to-back launches."""
+    import torch
+    g = capture(lambda: [fn() for _ in range(reps)])
+    s = torch.cuda.current_stream()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    g.replay()
+    a.record(s)
+    g.replay()
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def oracle_levels(levels, T0=None, dtype=None):
+    """The oracle GN over the same (GPU-prepared) level inputs: FP32 per-pixel
+    arithmetic with FP64 sums (the decision-matched oracle), or everything in
+    FP64 with dtype=float64."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc
+    T = np.eye(4) if T0 is None else T0
+    res = None
+    for est, L in levels:
+        k = np.array([[L.fx, 0, L.cx], [0, L.fy, L.cy], [0, 0, 1]], np.float32)
+        c = lambda t: t.detach().float().cpu().numpy()
+        res = orc.icp_estimate(
+            k, c(L.src_points), c(L.src_normals), L.src_mask.cpu().numpy(),
+            c(L.tgt_points), c(L.tgt_normals), L.tgt_mask.cpu().numpy(),
+            est.num_iters,
+            src_feats=c(L.src_feats) if L.has_features else None,
+            tgt_feats=c(L.tgt_feats) if L.has_features else None, transform=T,
+            geom_weight=est.geom_weight, feat_weight=est.feat_weight,
+            distance_threshold=est.distance_threshold,
+            normals_angle_thresh=est.normals_angle_thresh,
+            feat_residual_thresh=est.feat_residual_thresh,
+            dtype=dtype or np.float32)
+        T = res["transform"] if res["transform"] is not None else np.eye(4)
+    return res
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(level, est, seconds):
+    """Reference-structured CPU path (OpenMP per-pixel loops, mutex scatter,
+    materialised partials, FP32 sums, FP64 solve) timed on this host: a
+    bounded sample of 640x480 GN iterations at all threads, then at one."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc
+    c = lambda t: t.detach().cpu().numpy()
+    k = np.array([[level.fx, 0, level.cx], [0, level.fy, level.cy], [0, 0, 1]],
+                 np.float32)
+    args = (k, c(level.src_points), c(level.src_normals), c(level.src_mask),
+            c(level.tgt_points), c(level.tgt_normals), c(level.tgt_mask))
+    kw = dict(src_feats=c(level.src_feats), tgt_feats=c(level.tgt_feats),
+              geom_weight=est.geom_weight, feat_weight=est.feat_weight,
+              distance_threshold=est.distance_threshold,
+              normals_angle_thresh=est.normals_angle_thresh,
+              feat_residual_thresh=est.feat_residual_thresh,
+              sum_dtype=np.float32, parallel=True, with_information=False)
+
+    def run(limit_s, min_iters):
+        orc.icp_estimate(*args, 1, **kw)  # warm-up
+        iters, t0, T = 0, time.perf_counter(), np.eye(4)
+        while True:
+            r = orc.icp_estimate(*args, 1, transform=T, **kw)
+            if r["transform"] is not None:
+                T = r["transform"]
+            iters += 1
+            el = time.perf_counter() - t0
+            if el >= limit_s and iters >= min_iters:
+                return iters / el, iters, el
+
+    threads = orc.num_threads()
+    rate, iters, el = run(seconds, 3)
+    orc.set_num_threads(1)
+    rate1, iters1, el1 = run(seconds / 3, 2)
+    orc.set_num_threads(threads)
+    return dict(rate=rate, iters=iters, el=el, threads=threads, rate1=rate1,
+                iters1=iters1, el1=el1)
+
+
+def registration_section(args, ms, levels, run, top, pair):
+    """The 3-level registration of the pair in one hipGraph: ms per pair and
+    its pose error against the ground truth and the oracles."""
+    import numpy as np
+    from slamtb import metrics
+    from slamtb.registration.icp import result_from_state
+    log("full 3-level registration (graph)")
+    g = capture(lambda: ms.enqueue_levels(run, levels))
+    reg_ms = time_graph(g, 50) * 1e3
+    result = result_from_state(run.read_state(), top.n, True)
+    T_gpu = result.transform.numpy()
+    out = {"registration_ms_per_pair": reg_ms,
+           "pose_err_vs_gt": list(metrics.pose_error(pair["T_gt"], T_gpu)),
+           "match_ratio": result.match_ratio}
+    if not args.no_oracle:
+        log("oracle registrations on the same inputs (FP32 pixels, then FP64)")
+        ref = oracle_levels(levels)
+        out["pose_err_vs_oracle"] = list(metrics.pose_error(ref["transform"], T_gpu))
+        ref64 = oracle_levels(levels, dtype=np.float64)
+        out["pose_err_vs_oracle_fp64"] = list(
+            metrics.pose_error(ref64["transform"], T_gpu))
+    return out
+
+
 def batched_section(seed, dev, npairs, tune):
-    """64 pairs drawn from {160x120, 320x240, 640x480, 1280x720}, 3 levels
-    each (10/5/3 it, geometric + 3-ch features): one batched hipGraph vs the
-    same pairs one after another through the single-pair path (also one
+    """npairs pairs drawn from {160x120, 320x240, 640x480, 1280x720}, 3 levels
+    each (10/5/3 it, geometric + 3-ch features): one batched hipGraph against
+    the same pairs one after another through the single-pair path (also one
     hipGraph)."""
     import numpy as np
-    import torch
     from slamtb import synthetic, metrics
     from slamtb.registration import ICPOdometry, MultiscaleRegistration
     from slamtb.registration.batch import BatchedMultiscaleICP
@@ -245,11 +304,9 @@ def batched_section(seed, dev, npairs, tune):
     rng = np.random.default_rng(seed + 1000)
     choices = [(160, 120), (320, 240), (640, 480), (1280, 720)]
     sizes = [choices[i] for i in rng.integers(0, 4, npairs)]
-    kw = dict(geom_weight=10.0, feat_weight=1.0, distance_threshold=0.1,
-              normals_angle_thresh=math.radians(20), feat_residual_thresh=2.75)
-    ms = MultiscaleRegistration([(1.0, ICPOdometry(10, **kw)),
-                                 (0.5, ICPOdometry(5, **kw)),
-                                 (0.5, ICPOdometry(3, **kw))])
+    ms = MultiscaleRegistration([(1.0, ICPOdometry(10, **KW)),
+                                 (0.5, ICPOdometry(5, **KW)),
+                                 (0.5, ICPOdometry(3, **KW))])
     pairs, truths = [], []
     for i, (W, H) in enumerate(sizes):
         pr = synthetic.make_pair(seed + 100 + i, W, H,
@@ -264,8 +321,6 @@ def batched_section(seed, dev, npairs, tune):
     g_batch = capture(batch.enqueue)
     t_batch = time_graph(g_batch, 5)
     res_b = batch.results()
-    # Plain: every pair through the single-pair fused path, one after the
-    # other (its own runner per pair so its result stays readable).
     runners = []
     for levels in pairs:
         r = GNRunner(dev)
@@ -289,6 +344,7 @@ def batched_section(seed, dev, npairs, tune):
         gt_r.append(e[0])
         gt_t.append(e[1])
     errs = np.array(errs)
+    px_iters = sum(L.n * est.num_iters for levels in pairs for est, L in levels)
     return {
         "pairs": npairs, "tile_px": batch.plans[0].sizes.tile_px,
         "grid": batch.plans[0].params.grid_update,
@@ -296,6 +352,7 @@ def batched_section(seed, dev, npairs, tune):
         "plain_pairs_per_s": npairs / t_plain,
         "speedup_vs_plain": t_plain / t_batch,
         "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3,
+        "batched_gpx_iters_per_s": px_iters / t_batch / 1e9,
         "max_pose_err_vs_single_pair": [float(np.nanmax(errs[:, 0])),
                                          float(np.nanmax(errs[:, 1]))],
         "median_pose_err_vs_gt": [float(np.median(gt_r)), float(np.median(gt_t))],
@@ -306,19 +363,16 @@ def batched_section(seed, dev, npairs, tune):
 def bf16_section(seed, dev, with_oracle):
     """1280x720, 4 levels (10/5/3/2 it), geometric + 32-channel unit
     descriptors; BF16 storage against FP32 storage of the same scene."""
-    import numpy as np
     import torch
     from slamtb import synthetic, metrics
     from slamtb.registration import ICPOdometry, MultiscaleRegistration
     from slamtb.registration._fused import GNRunner
     from slamtb.registration.icp import result_from_state
 
-    kw = dict(geom_weight=10.0, feat_weight=1.0, distance_threshold=0.1,
-              normals_angle_thresh=math.radians(20), feat_residual_thresh=2.75)
-    ms = MultiscaleRegistration([(1.0, ICPOdometry(10, **kw)),
-                                 (0.5, ICPOdometry(5, **kw)),
-                                 (0.5, ICPOdometry(3, **kw)),
-                                 (0.5, ICPOdometry(2, **kw))])
+    ms = MultiscaleRegistration([(1.0, ICPOdometry(10, **KW)),
+                                 (0.5, ICPOdometry(5, **KW)),
+                                 (0.5, ICPOdometry(3, **KW)),
+                                 (0.5, ICPOdometry(2, **KW))])
     pr = synthetic.make_pair(seed + 500, 1280, 720,
                              kcam=synthetic.intrinsics(1280, 720), channels=32,
                              descriptor=True)
@@ -355,38 +409,72 @@ def bf16_section(seed, dev, with_oracle):
     return out
 
 
-def registration_section(args, ms, levels, run, top, pair):
-    """The full 3-level registration of the pair captured in one hipGraph:
-    ms per pair, the result and its pose error against the ground truth and
-    the oracle fed the same level inputs."""
+def diff_section(seed, dev, batch=8, steps=5):
+    """5 unrolled GN steps on `batch` pairs at 640x480 (geometric + 3-channel
+    features), forward and the adjoint backward of the squared geodesic pose
+    loss to the ground truth, with respect to the source depth, target points
+    and normals, both feature maps and the initial pose."""
+    import numpy as np
     import torch
-    from slamtb import metrics
-    from slamtb.registration.icp import result_from_state
-    log("full 3-level registration (graph)")
-    reg_graph = torch.cuda.CUDAGraph()
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        ms.enqueue_levels(run, levels)  # warm the sequence once uncaptured
-        with torch.cuda.graph(reg_graph, stream=side):
-            ms.enqueue_levels(run, levels)
-    torch.cuda.current_stream().wait_stream(side)
+    from slamtb import synthetic
+    from slamtb.camera import KCamera
+    from slamtb.frame import Frame, FrameInfo, FramePointCloud
+    from slamtb.registration.diff import DifferentiableICP, pose_loss
+
+    k = synthetic.intrinsics(640, 480).astype(np.float32)
+    kc = KCamera(torch.from_numpy(k))
+    pairs = [synthetic.make_pair(seed + 700 + i, 640, 480, kcam=k, channels=3)
+             for i in range(batch)]
+
+    def fpc(depth):
+        return FramePointCloud.from_frame(
+            Frame(FrameInfo(kc), depth_image=torch.from_numpy(depth).to(dev)),
+            device=dev)
+    src = [fpc(p["src_depth"]) for p in pairs]
+    tgt = [fpc(p["tgt_depth"]) for p in pairs]
+    st = lambda xs: torch.stack(xs).contiguous()
+    depth = st([torch.from_numpy(p["src_depth"]) for p in pairs]).to(dev)
+    src_n, src_m = st([f.normals for f in src]), st([f.mask for f in src])
+    tgt_p, tgt_n = st([f.points for f in tgt]), st([f.normals for f in tgt])
+    tgt_m = st([f.mask for f in tgt])
+    sf = st([torch.from_numpy(p["src_feats"]) for p in pairs]).to(dev)
+    tf = st([torch.from_numpy(p["tgt_feats"]) for p in pairs]).to(dev)
+    T_gt = torch.from_numpy(np.stack([p["T_gt"] for p in pairs])).to(dev)
+    leaves = [depth, tgt_p, tgt_n, sf, tf]
+    for t in leaves:
+        t.requires_grad_()
+    x0 = torch.zeros(batch, 6, dtype=torch.float64, device=dev, requires_grad=True)
+    icp = DifferentiableICP(steps, **KW)
+
+    def forward():
+        return icp(k, depth, src_n, src_m, tgt_p, tgt_n, tgt_m, x0,
+                   src_feats=sf, tgt_feats=tf)
+
+    def step():
+        for t in leaves + [x0]:
+            t.grad = None
+        pose_loss(forward(), T_gt).sum().backward()
+
+    step()
     torch.cuda.synchronize()
-    reps_reg = 50
-    t0 = time.perf_counter()
-    for _ in range(reps_reg):
-        reg_graph.replay()
-    torch.cuda.synchronize()
-    reg_ms = (time.perf_counter() - t0) * 1e3 / reps_reg
-    result = result_from_state(run.read_state(), top.n, True)
-    T_gpu = result.transform.numpy()
-    err_gt = metrics.pose_error(pair["T_gt"], T_gpu)
-    err_or = None
-    if not args.no_oracle:
-        log("oracle registration on the same inputs")
-        ref = oracle_levels(levels)
-        err_or = metrics.pose_error(ref["transform"], T_gpu)
-    return reg_ms, result, list(err_gt), err_or
+    fwd, tot = [], []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            forward()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        tot.append(t2 - t1)
+    ms_b = statistics.median(tot) * 1e3
+    return {"batch": batch, "steps": steps, "resolution": "640x480",
+            "forward_ms": statistics.median(fwd) * 1e3,
+            "forward_backward_ms": ms_b, "pairs_per_s": batch / ms_b * 1e3,
+            "grad_finite": bool(all(torch.isfinite(t.grad).all().item()
+                                    for t in leaves + [x0]))}
 
 
 def main():
@@ -401,100 +489,89 @@ def main():
         sys.exit(2)
     import torch
     from slamtb.registration._fused import runner_for
+    if args.profile:
+        args.no_oracle = args.no_cpu_baseline = args.no_bf16 = True
+        args.no_diff = True
+        args.pairs = 0
 
     dev = torch.device("cuda:0")
     log("preparing 640x480 pair on the GPU")
     pair, ms, levels = prepare(args.seed, dev)
     est, top = levels[-1]
     params = est.params(top.has_features)
+    if args.tune:
+        from slamtb import tuning
+        params = tuning.tune_level(top, params)
+        log(f"tuned grids: update {params.grid_update} corresp {params.grid_corresp}")
     run = runner_for(dev)
     for _, L in levels:
         run.reserve(L.n, L.height, L.width)
 
-    # --- warm-up and timed steps -----------------------------------------
+    # --- warm-up and the timed steps ---------------------------------------
     log(f"warm-up {args.warmup} steps")
     run.begin(top)
     run.iterate(top, params, args.warmup)
     torch.cuda.synchronize()
     K = args.steps
-    graph = None
-    if not args.no_graph:
-        graph = torch.cuda.CUDAGraph()
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):
-            run.begin(top)
-            with torch.cuda.graph(graph, stream=side):
-                run.iterate(top, params, K)
-        torch.cuda.current_stream().wait_stream(side)
-        torch.cuda.synchronize()
+    graph = None if args.no_graph else capture(
+        lambda: (run.begin(top), run.iterate(top, params, K)))
     log(f"timing {K} steps ({'graph' if graph else 'stream'})")
-    run.begin(top)
+
+    def timed():
+        if graph is not None:
+            graph.replay()
+        else:
+            run.begin(top)
+            run.iterate(top, params, K)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    if graph is not None:
-        graph.replay()
-    else:
-        run.iterate(top, params, K)
+    timed()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st = run.read_state()
-    ms_per_step = (t1 - t0) * 1e3 / K
     value = K / (t1 - t0)
+    ms_per_step = (t1 - t0) * 1e3 / K
     steps_ok = (st.stopped == 0 and st.iterations == K)
+    reruns = []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        timed()
+        torch.cuda.synchronize()
+        reruns.append((time.perf_counter() - a) * 1e3 / K)
 
-    # --- per-kernel timing for the roofline ----------------------------------
-    log("per-kernel event timing")
+    # --- per-launch durations and the roofline -------------------------------
+    log("per-launch event timing")
     run.begin(top)
     run.iterate(top, params, 2)
-    reps = max(10, min(200, K)) if args.profile else 200
-    (a_avg, a_med), (b_only, b_med) = time_iteration_kernels(run, top, params, reps)
+    reps = 100
+    corr_ms = launch_ms(lambda: run.correspond(top, params), reps)
+    jac_ms = launch_ms(lambda: run.jacobians(top, params), reps)
+    run.begin(top)
+    run.iterate(top, params, 2)
     st2 = run.read_state()
     n = top.n
     s, C = 4, top.channels
     valid = float(top.src_mask.float().mean().item())
     matched = max(st2.last_geom_count, st2.last_feat_count) / n
+    # Algorithmic bytes per launch: all-pixel figures (12s+18, 9s+9+2Cs B/px)
+    # and the honest ones: mask bytes for every pixel, source and gather
+    # bytes for valid pixels, Jacobian gathers for matched pixels.
     bytes_a_all = (12 * s + 18) * n
     bytes_b_all = (9 * s + 9 + 2 * C * s) * n
-    # Valid/matched-scaled ("honest") bytes: mask bytes for every pixel,
-    # source and gather bytes for valid pixels, Jacobian gathers for matches.
     bytes_a = n * (1 + 8 + valid * (6 * s + 1 + 6 * s + 8))
     bytes_b = n * (1 + valid * (3 * s + 8) + matched * (6 * s + 2 * C * s))
-    a_gbs = bytes_a / (a_avg * 1e-3) / 1e9
-    b_gbs = bytes_b / (b_only * 1e-3) / 1e9
-    if a_avg >= b_only:
-        dom = dict(kernel="corresp_fused_kernel", ms=a_avg, gbs=a_gbs,
-                   bytes=bytes_a, bytes_all=bytes_a_all)
+    if corr_ms >= jac_ms:
+        dom = dict(kernel="corresp_fused_kernel", ms=corr_ms, bytes=bytes_a,
+                   bytes_all=bytes_a_all)
     else:
-        dom = dict(kernel="jacobian_fused_kernel", ms=b_only, gbs=b_gbs,
-                   bytes=bytes_b, bytes_all=bytes_b_all)
-
-    if args.profile:
-        args.no_oracle = args.no_cpu_baseline = args.no_bf16 = True
-        args.pairs = 0
-    reg_ms, result, err_gt, err_or = None, None, None, None
-    if not args.profile:
-        reg_ms, result, err_gt, err_or = registration_section(
-            args, ms, levels, run, top, pair)
-
-    batched = None
-    if args.pairs > 0:
-        log(f"batched workload: {args.pairs} pairs")
-        batched = batched_section(args.seed, dev, args.pairs, not args.no_tune)
-    bf16 = None
-    if not args.no_bf16:
-        log("1280x720 BF16 / FP32 storage workload")
-        bf16 = bf16_section(args.seed, dev, not args.no_oracle)
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")
-        rate, iters, el, thr = cpu_baseline(top, est, args.cpu_seconds)
-        cpu = {"value": rate, "unit": "GN iters/s", "cores": thr,
-               "kind": "port",
-               "sample": f"{iters} GN iterations at 640x480 (geometric + 3-ch "
-                         f"feature, FP32) of the oracle's OpenMP port of the "
-                         f"reference CPU path in {el:.1f} s on {thr} threads"}
+        dom = dict(kernel="jacobian_fused_kernel", ms=jac_ms, bytes=bytes_b,
+                   bytes_all=bytes_b_all)
+    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    traffic = None
+    if TRAFFIC_FILE.exists() and dom["kernel"] == "jacobian_fused_kernel":
+        traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
+    _, copy_gbs = copy_rate(256)
 
     out = {
         "metric": METRIC, "value": value, "unit": "GN iters/s", "n_gpus": 1,
@@ -504,23 +581,47 @@ def main():
         "config": {"workload": "icp-gn-640x480-geom+feat3-fp32",
                    "resolution": "640x480", "feature_channels": C,
                    "levels": "640x480/320x240/160x120 10/5/3 it",
-                   "weights": "geom 10 : feat 1 (the 1:0.1 ratio in the reference preset form)", "timing": "hipGraph" if graph else "stream",
+                   "weights": "geom 10 : feat 1 (the 1:0.1 ratio in the reference preset form)",
+                   "timing": "hipGraph" if graph else "stream",
+                   "grid_update": params.grid_update,
+                   "grid_corresp": params.grid_corresp,
                    "steps_all_accepted": steps_ok},
-        "roofline": {"bound": "hbm", "achieved": dom["gbs"],
+        "ms_per_step_median_of_5": statistics.median(reruns),
+        "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": dom["kernel"], "kernel_ms": dom["ms"],
                      "bytes_per_launch": dom["bytes"],
-                     "achieved_all_pixels": dom["bytes_all"] / (dom["ms"] * 1e-3) / 1e9},
-        "kernels_ms": {"corresp_fused_kernel": a_avg, "jacobian_fused_kernel": b_only,
-                       "corresp_median": a_med, "jacobian_median": b_med},
-        "registration_ms_per_pair": reg_ms,
-        "pose_err_vs_oracle": err_or, "pose_err_vs_gt": err_gt,
-        "match_ratio": None if result is None else result.match_ratio,
-        "batched": batched,
-        "bf16_1280x720": bf16,
-        "cpu_baseline": cpu,
+                     "bytes_per_launch_all_pixels": dom["bytes_all"],
+                     "achieved_all_pixels": dom["bytes_all"] / (dom["ms"] * 1e-3) / 1e9,
+                     "copy_ceiling_gbs": copy_gbs},
+        "launch_ms": {"corresp_fused_kernel": corr_ms,
+                      "jacobian_fused_kernel": jac_ms,
+                      "reduce_solve_and_gaps": statistics.median(reruns) - corr_ms - jac_ms},
     }
+    if not args.profile:
+        out.update(registration_section(args, ms, levels, run, top, pair))
+    if args.pairs > 0:
+        log(f"batched workload: {args.pairs} pairs")
+        out["batched"] = batched_section(args.seed, dev, args.pairs, not args.no_tune)
+    if not args.no_bf16:
+        log("1280x720 BF16 / FP32 storage workload")
+        out["bf16_1280x720"] = bf16_section(args.seed, dev, not args.no_oracle)
+    if not args.no_diff:
+        log("differentiable GN: 8 pairs at 640x480, 5 steps, forward + backward")
+        out["diff_8x640x480"] = diff_section(args.seed, dev)
+    if not args.no_cpu_baseline:
+        log(f"CPU baseline (~{args.cpu_seconds * 4 / 3:.0f} s)")
+        cb = cpu_baseline(top, est, args.cpu_seconds)
+        out["cpu_baseline"] = {
+            "value": cb["rate"], "unit": "GN iters/s", "cores": cb["threads"],
+            "kind": "port",
+            "sample": f"{cb['iters']} GN iterations at 640x480 (geometric + "
+                      f"3-ch feature, FP32) of the oracle's OpenMP port of the "
+                      f"reference CPU path in {cb['el']:.1f} s on {cb['threads']} "
+                      f"threads of {cpu_model()}",
+            "one_thread": {"value": cb["rate1"], "iters": cb["iters1"],
+                           "seconds": cb["el1"]}}
     print(json.dumps(out), flush=True)
 
 

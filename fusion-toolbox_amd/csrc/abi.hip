@@ -100,7 +100,9 @@ double angle_gate_cos_f64(double angle_thresh) {
 
 extern "C" const char* stb_last_error(void) { return stb::g_error; }
 
-extern "C" int stb_abi_version(void) { return 1; }
+extern "C" int stb_abi_version(void) { return 2; }
+
+extern "C" size_t stb_icp_state_bytes(void) { return sizeof(stb_icp_state); }
 
 // Exposed for tests: the cosine thresholds used by the correspondence gate.
 extern "C" float stb_angle_gate_cos_f32(double angle_thresh) {

@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kBlock)
 // Workspace layout: [state | map0 | map1 | partials]
 // ---------------------------------------------------------------------------
 
-constexpr int kDefaultUpdateBlocks = 512;
+constexpr int kDefaultUpdateBlocks = 1024;
 constexpr int kMaxUpdateBlocks = 2048;
 constexpr int kSolveGroups = 16;
 constexpr int kDefaultCorrespBlocks = 4096;
@@ -226,7 +226,7 @@ int check_level(const stb_icp_level* L, bool need_feats) {
 
 // Phases of one GN iteration. kCorrespond and kUpdate are the two kernels of
 // the loop; kLinearize and kInformation are one-off passes on the spare map.
-enum Phase { kCorrespond, kUpdate, kLinearize, kInformation };
+enum Phase { kCorrespond, kUpdate, kLinearize, kInformation, kJacobians };
 
 template <typename S, bool GEOM, bool FEAT>
 void launch_jacobian(const stb_icp_level& L, const stb_icp_params& P,
@@ -236,6 +236,7 @@ void launch_jacobian(const stb_icp_level& L, const stb_icp_params& P,
   hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT>), dim3(grid),
                      dim3(kBlock), 0, stream, L, w.st, w.map[0], w.map[1],
                      w.map_len, w.partials, tw, fixed_map);
+  if (do_solve < 0) return;  // the Jacobian launch alone
   const int groups = grid < kSolveGroups ? grid : kSolveGroups;
   hipLaunchKernelGGL(solve_fused_kernel, dim3(groups), dim3(kBlock), 0, stream,
                      w.st, w.partials, grid, w.groups, tw.gw, tw.fw, do_solve,
@@ -247,13 +248,15 @@ void launch_phase(const stb_icp_level& L, const stb_icp_params& P,
                   const Workspace& w, Phase phase, hipStream_t stream) {
   const float dthr2 = float(P.distance_thresh * P.distance_thresh);
   const float cstar = angle_gate_cos_f32(P.normal_angle_thresh);
-  const int fixed = (phase == kLinearize || phase == kInformation) ? 1 : 0;
-  if (fixed) {
+  const int fixed =
+      (phase == kLinearize || phase == kInformation || phase == kJacobians) ? 1
+                                                                            : 0;
+  if (fixed && phase != kJacobians) {
     hipLaunchKernelGGL(fill_spare_map_kernel,
                        dim3(grid_for(w.map_len, kBlock, 2048)), dim3(kBlock), 0,
                        stream, w.st, w.map[0], w.map[1], w.map_len);
   }
-  if (phase != kUpdate) {
+  if (phase != kUpdate && phase != kJacobians) {
     hipLaunchKernelGGL(corresp_fused_kernel<S>, dim3(corresp_grid(P, L.n_src)),
                        dim3(kBlock), 0, stream, L, w.st, w.map[0], w.map[1],
                        dthr2, cstar, fixed);
@@ -266,7 +269,7 @@ void launch_phase(const stb_icp_level& L, const stb_icp_params& P,
     return;
   }
   const TermWeights tw = make_weights(P);
-  const int solve = phase == kUpdate ? 1 : 0;
+  const int solve = phase == kUpdate ? 1 : (phase == kJacobians ? -1 : 0);
   if (P.use_geom && P.use_feat)
     launch_jacobian<S, true, true>(L, P, w, tw, solve, fixed, stream);
   else if (P.use_geom)
@@ -291,7 +294,8 @@ int run_fused(const stb_icp_level* level, const stb_icp_params* params,
               void* stream, const char* what) {
   STB_REQUIRE(params != nullptr && workspace != nullptr, "%s: NULL argument",
               what);
-  const bool jac = phase == kUpdate || phase == kLinearize || phase < 0;
+  const bool jac = phase == kUpdate || phase == kLinearize ||
+                   phase == kJacobians || phase < 0;
   int rc = check_level(level, jac && params->use_feat != 0);
   if (rc) return rc;
   if (jac) {
@@ -378,6 +382,13 @@ extern "C" int stb_icp_update(const stb_icp_level* level,
                               size_t workspace_bytes, void* stream) {
   return run_fused(level, params, workspace, workspace_bytes, kUpdate, 1,
                    stream, "stb_icp_update");
+}
+
+extern "C" int stb_icp_jacobians(const stb_icp_level* level,
+                                 const stb_icp_params* params, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
+  return run_fused(level, params, workspace, workspace_bytes, kJacobians, 1,
+                   stream, "stb_icp_jacobians");
 }
 
 extern "C" int stb_icp_linearize(const stb_icp_level* level,

@@ -91,35 +91,36 @@ __device__ inline LevelView<S> view_of(const stb_icp_level& L) {
 // Correspondence of source pixel idx (src/correspondence_map.cu:38-61 with
 // RobustCorrespondence::Match): transform, project, round, bounds, target
 // mask, distance and normal-angle gates, then a 64-bit unsigned atomic min of
-// (bits(distance) << 32 | idx) into the map.
+// (bits(distance) << 32 | idx) into the map. Each test is written after the
+// loads it depends on and reads their results, so the compiler cannot sink a
+// load below an earlier test: the source loads issue together, then the
+// target loads, and a pixel costs two memory round trips.
 template <typename S>
 __device__ inline void corresp_pixel(const LevelView<S>& L, const float* m,
                                      const float* nm, int64_t idx,
                                      unsigned long long* __restrict__ map,
                                      float dthr2, float cstar) {
 #pragma clang fp contract(off)
-  // Mask, point and normal loads issue together; idx < n keeps them in
-  // bounds whatever the mask says.
   const uint8_t valid_src = L.sm[idx];
   const V3<float> ps = ld3f(L.sp, idx);
   const V3<float> ns = ld3f(L.sn, idx);
-  if (!valid_src) return;
   const V3<float> p = rigid_point(m, ps);
   const V3<float> nn = rigid_normal(nm, ns);
   float u, v;
   project(L.fx, L.fy, L.cx, L.cy, p, u, v);
   int ui, vi;
-  if (!round_in_range(u, L.W, ui) || !round_in_range(v, L.H, vi)) return;
+  const bool in_u = round_in_range(u, L.W, ui);
+  const bool in_v = round_in_range(v, L.H, vi);
+  if (!valid_src || !in_u || !in_v) return;
   const int64_t pos = int64_t(vi) * L.W + ui;
-  // Issue the three target loads together; the gates then run on registers.
   const uint8_t valid = L.tm[pos];
   const V3<float> q = ld3f(L.tp, pos);
   const V3<float> qn = ld3f(L.tn, pos);
-  if (!valid) return;
   const V3<float> d = sub3(q, p);
   const float sq = dot3(d, d);
-  if (sq > dthr2) return;
-  if (angle_rejects(gate_cos(nn, qn), cstar)) return;
+  const bool keep = (valid != 0) & !(sq > dthr2) &
+                    !angle_rejects(gate_cos(nn, qn), cstar);
+  if (!keep) return;
   atomicMin(map + pos, (unsigned long long)make_key(sqrtf(sq), uint32_t(idx)));
 }
 
@@ -143,7 +144,12 @@ __device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
 
 // Point-to-plane and feature terms of source pixel idx when the map names it
 // as its target pixel's owner (src/icp_jacobian_geometric.cu:54-85,
-// src/icp_jacobian_feature.cu:58-109).
+// src/icp_jacobian_feature.cu:58-109). The owner word, the target point and
+// normal and the feature taps around (u, v) are loaded whatever the owner
+// test says and the terms are accumulated through selects, so all of those
+// loads issue in one batch: a pixel costs two memory round trips. A pixel
+// that does not own its target adds exact zeros (selected, never multiplied
+// in, so NaN or inf in its unused data cannot leak into the sums).
 template <typename S, bool GEOM, bool FEAT>
 __device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
                                       const unsigned long long* __restrict__ map,
@@ -152,40 +158,43 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
 #pragma clang fp contract(off)
   const uint8_t valid_src = L.sm[idx];
   const V3<float> ps = ld3f(L.sp, idx);
-  if (!valid_src) return;
   const V3<float> p = rigid_point(m, ps);
   float u, v;
   project(L.fx, L.fy, L.cx, L.cy, p, u, v);
   int ui, vi;
-  if (!round_in_range(u, L.W, ui) || !round_in_range(v, L.H, vi)) return;
+  const bool in_u = round_in_range(u, L.W, ui);
+  const bool in_v = round_in_range(v, L.H, vi);
+  if (!valid_src || !in_u || !in_v) return;
   const int64_t pos = int64_t(vi) * L.W + ui;
-  // The owner word and the target point/normal sit at the same in-range
-  // pixel: load them together so the owner test does not add a round trip.
-  const uint32_t owner = uint32_t(map[pos]);
-  V3<float> q, qn;
+  // Only the owner half of the key (the low word: idx) is read. Loading the
+  // whole key would leave its unused high half's register free for reuse,
+  // which makes the compiler wait for the load before anything else issues.
+  const uint32_t owner = reinterpret_cast<const uint32_t*>(map)[2 * pos];
+  const bool own = owner == uint32_t(idx);
   if (GEOM) {
-    q = ld3f(L.tp, pos);
-    qn = ld3f(L.tn, pos);
-  }
-  if (owner != uint32_t(idx)) return;
-  if (GEOM) {
+    const V3<float> q = ld3f(L.tp, pos);
+    const V3<float> qn = ld3f(L.tn, pos);
     const float r = dot3(sub3(q, p), qn);
     const V3<float> c = cross3(p, qn);
-    const float j[6] = {qn.x, qn.y, qn.z, c.x, c.y, c.z};
-    acc_term(acc, j, tw.wg, r * tw.sg);
-    acc[27] = fmaf(r, r, acc[27]);
-    acc[29] += 1.0f;
+    const float j[6] = {own ? qn.x : 0.0f, own ? qn.y : 0.0f, own ? qn.z : 0.0f,
+                        own ? c.x : 0.0f,  own ? c.y : 0.0f,  own ? c.z : 0.0f};
+    const float rs = own ? r : 0.0f;
+    acc_term(acc, j, tw.wg, rs * tw.sg);
+    acc[27] = fmaf(rs, rs, acc[27]);
+    acc[29] += own ? 1.0f : 0.0f;
   }
   if (FEAT) {
-    float j[6], rl;
+    float jf[6], rl;
     const float sq = feature_terms<float, S, S>(L.tf, L.C, L.H, L.W, L.HW, L.sf,
                                                 L.n, idx, L.fx, L.fy, p, u, v,
-                                                j, rl);
-    if (!(sq > tw.fthr2)) {
-      acc_term(acc, j, tw.wf, rl * tw.sf);
-      acc[28] += sq;
-      acc[30] += 1.0f;
-    }
+                                                jf, rl);
+    const bool use = own && !(sq > tw.fthr2);
+    float j[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) j[a] = use ? jf[a] : 0.0f;
+    acc_term(acc, j, tw.wf, (use ? rl : 0.0f) * tw.sf);
+    acc[28] += use ? sq : 0.0f;
+    acc[30] += use ? 1.0f : 0.0f;
   }
 }
 
@@ -429,6 +438,10 @@ __device__ inline void gn_finalize(stb_icp_state* st, const double* tot,
     for (int k = 0; k < 6; ++k) b[k] = tot[21 + k];
 #pragma unroll
     for (int k = 0; k < 12; ++k) T[k] = w[k];
+    // L's diagonal enters only through its reciprocals rinv[j] = 1/sqrt(d2):
+    // the hardware reciprocal square root refined by two Newton steps
+    // (relative error ~1e-16, like sqrt followed by a divide, but a
+    // dependent chain of 7 instructions instead of ~25).
     double L[21], rinv[6];
     bool ok = true;
 #pragma unroll
@@ -437,9 +450,10 @@ __device__ inline void gn_finalize(stb_icp_state* st, const double* tot,
 #pragma unroll
       for (int q = 0; q < j; ++q) d2 -= L[tri_lo(j, q)] * L[tri_lo(j, q)];
       ok = ok && (d2 > 0.0);
-      const double d = sqrt(d2);
-      L[tri_lo(j, j)] = d;
-      rinv[j] = 1.0 / d;
+      double r = __builtin_amdgcn_rsq(d2);
+      r = fma(r * 0.5, fma(-d2 * r, r, 1.0), r);
+      r = fma(r * 0.5, fma(-d2 * r, r, 1.0), r);
+      rinv[j] = r;
 #pragma unroll
       for (int i = j + 1; i < 6; ++i) {
         double v = A[tri_up(j, i)];

@@ -62,19 +62,47 @@ __device__ inline Bil<T> make_bil(T u, T v) {
   return b;
 }
 
+// The four taps are loaded unconditionally (a tap past the right or bottom
+// edge re-reads an in-range texel and is then replaced by zero), so they
+// issue together with no branches between them. The cell (ui, vi) is in
+// range for every caller: u and v lie in [-0.5, W - 0.5) x [-0.5, H - 0.5)
+// (plus at most h = 0.05), which truncates into the image.
 template <typename T, typename S>
-__device__ inline T bil_get(const S* __restrict__ ch, int H, int W,
-                            const Bil<T>& b) {
-#pragma clang fp contract(off)
+__device__ inline void bil_taps(const S* __restrict__ ch, int H, int W,
+                                const Bil<T>& b, T t[4]) {
   const int64_t o = int64_t(b.vi) * W + b.ui;
   const bool rx = b.ui + 1 < W, ry = b.vi + 1 < H;
-  const T v00 = T(ldw(ch, o));
-  const T v10 = rx ? T(ldw(ch, o + 1)) : T(0);
-  const T v01 = ry ? T(ldw(ch, o + W)) : T(0);
-  const T v11 = (rx && ry) ? T(ldw(ch, o + W + 1)) : T(0);
+  const int64_t dx = rx ? 1 : 0, dy = ry ? W : 0;
+  t[0] = T(ldw(ch, o));
+  t[1] = T(ldw(ch, o + dx));
+  t[2] = T(ldw(ch, o + dy));
+  t[3] = T(ldw(ch, o + dy + dx));
+}
+
+template <typename T>
+__device__ inline T bil_mix(int H, int W, const Bil<T>& b, const T t[4]) {
+#pragma clang fp contract(off)
+  const bool rx = b.ui + 1 < W, ry = b.vi + 1 < H;
+  const T v00 = t[0];
+  const T v10 = rx ? t[1] : T(0);
+  const T v01 = ry ? t[2] : T(0);
+  const T v11 = (rx && ry) ? t[3] : T(0);
   const T a = v00 * (T(1) - b.ur) + v10 * b.ur;
   const T c = v01 * (T(1) - b.ur) + v11 * b.ur;
   return a * (T(1) - b.vr) + c * b.vr;
+}
+
+// The four taps are loaded unconditionally (a tap past the right or bottom
+// edge re-reads an in-range texel and is then replaced by zero), so they
+// issue together with no branches between them. The cell (ui, vi) is in
+// range for every caller: u and v lie in [-0.5, W - 0.5) x [-0.5, H - 0.5)
+// (plus at most h = 0.05), which truncates into the image.
+template <typename T, typename S>
+__device__ inline T bil_get(const S* __restrict__ ch, int H, int W,
+                            const Bil<T>& b) {
+  T t[4];
+  bil_taps(ch, H, W, b, t);
+  return bil_mix(H, W, b, t);
 }
 
 // Feature residuals and the last channel's Jacobian at one matched pixel
@@ -90,19 +118,40 @@ __device__ inline T feature_terms(const S* __restrict__ tf, int C, int H, int W,
   const T h = T(0.05);
   const T div = T(1) / (T(2) * h);
   const Bil<T> c0 = make_bil(u, v);
-  T sq = T(0);
-  for (int c = 0; c < C - 1; ++c) {
-    const T res = bil_get(tf + c * HW, H, W, c0) - T(ldw(sf, c * n + idx));
-    sq += res * res;
-  }
-  const S* ch = tf + int64_t(C - 1) * HW;
-  const T res = bil_get(ch, H, W, c0) - T(ldw(sf, int64_t(C - 1) * n + idx));
   const Bil<T> bu0 = make_bil(u + h, v);
   const Bil<T> bu1 = make_bil(fmax(u - h, T(0)), v);
   const Bil<T> bv0 = make_bil(u, v + h);
   const Bil<T> bv1 = make_bil(u, fmax(v - h, T(0)));
-  const T du = (bil_get(ch, H, W, bu0) - bil_get(ch, H, W, bu1)) * div;
-  const T dv = (bil_get(ch, H, W, bv0) - bil_get(ch, H, W, bv1)) * div;
+  // The last channel's five samples come first in program order so their
+  // loads issue before the other channels' batches wait.
+  const S* ch = tf + int64_t(C - 1) * HW;
+  const T last = bil_get(ch, H, W, c0);
+  const T su0 = bil_get(ch, H, W, bu0), su1 = bil_get(ch, H, W, bu1);
+  const T sv0 = bil_get(ch, H, W, bv0), sv1 = bil_get(ch, H, W, bv1);
+  const T flast = T(ldw(sf, int64_t(C - 1) * n + idx));
+  // Channels 0 .. C-2 in batches of four whose loads issue together; a
+  // batch slot past C-2 re-reads channel C-2 and adds +0, which leaves the
+  // running sum (and the reference's summation order) unchanged.
+  T sq = T(0);
+  const int last_c = C - 2 > 0 ? C - 2 : 0;
+  for (int c0i = 0; c0i < C - 1; c0i += 4) {
+    // All sixteen taps and four source values are loaded before any is used.
+    T taps[4][4], fs[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int c = c0i + k < last_c ? c0i + k : last_c;
+      bil_taps(tf + c * HW, H, W, c0, taps[k]);
+      fs[k] = T(ldw(sf, c * n + idx));
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const T res = bil_mix(H, W, c0, taps[k]) - fs[k];
+      sq += (c0i + k < C - 1) ? res * res : T(0);
+    }
+  }
+  const T res = last - flast;
+  const T du = (su0 - su1) * div;
+  const T dv = (sv0 - sv1) * div;
   const T z2 = p.z * p.z;
   const T j00 = fx / p.z;
   const T j02 = -p.x * fx / z2;

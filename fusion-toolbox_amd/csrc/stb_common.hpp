@@ -155,13 +155,19 @@ template <>
 __device__ inline float ldf<float>(const float* p, int64_t i) {
   return p[i];
 }
+// 16-bit elements are loaded as raw bits and widened in registers: BF16 is
+// the high half of an FP32, FP16 converts with one instruction. (Loading them
+// through the library's struct types made the compiler wait on every load.)
+__device__ inline float bf16_bits_to_float(uint16_t b) {
+  return __uint_as_float(uint32_t(b) << 16);
+}
 template <>
 __device__ inline float ldf<__hip_bfloat16>(const __hip_bfloat16* p, int64_t i) {
-  return __bfloat162float(p[i]);
+  return bf16_bits_to_float(reinterpret_cast<const uint16_t*>(p)[i]);
 }
 template <>
 __device__ inline float ldf<__half>(const __half* p, int64_t i) {
-  return __half2float(p[i]);
+  return float(reinterpret_cast<const _Float16*>(p)[i]);
 }
 
 template <typename S>
@@ -182,11 +188,9 @@ __device__ inline V3<T> ld3(const T* p, int64_t i) {
 __device__ inline float ldw(const float* p, int64_t i) { return p[i]; }
 __device__ inline double ldw(const double* p, int64_t i) { return p[i]; }
 __device__ inline float ldw(const __hip_bfloat16* p, int64_t i) {
-  return __bfloat162float(p[i]);
+  return ldf(p, i);
 }
-__device__ inline float ldw(const __half* p, int64_t i) {
-  return __half2float(p[i]);
-}
+__device__ inline float ldw(const __half* p, int64_t i) { return ldf(p, i); }
 
 // ---------------------------------------------------------------------------
 // Correspondence keys.

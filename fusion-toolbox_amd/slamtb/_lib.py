@@ -81,6 +81,7 @@ _PARAMS = ctypes.POINTER(StbIcpParams)
 _SIGNATURES = {
     "stb_last_error": (ctypes.c_char_p, []),
     "stb_abi_version": (ctypes.c_int, []),
+    "stb_icp_state_bytes": (_sz, []),
     "stb_angle_gate_cos_f32": (_f32, [_f64]),
     "stb_angle_gate_cos_f64": (_f64, [_f64]),
     "stb_compute_correspondence_map": (ctypes.c_int, [
@@ -127,6 +128,9 @@ _SIGNATURES = {
         ctypes.POINTER(StbIcpLevel), ctypes.POINTER(StbIcpParams), _vp, _sz,
         _vp]),
     "stb_icp_update": (ctypes.c_int, [
+        ctypes.POINTER(StbIcpLevel), ctypes.POINTER(StbIcpParams), _vp, _sz,
+        _vp]),
+    "stb_icp_jacobians": (ctypes.c_int, [
         ctypes.POINTER(StbIcpLevel), ctypes.POINTER(StbIcpParams), _vp, _sz,
         _vp]),
     "stb_icp_linearize": (ctypes.c_int, [

@@ -148,6 +148,14 @@ class GNRunner:
                 ctypes.byref(level.struct), ctypes.byref(params), ws, nbytes,
                 _lib.stream()), "stb_icp_linearize")
 
+    def jacobians(self, level, params):
+        """The Jacobian launch alone over the last iteration's map (timing)."""
+        ws, nbytes = self._ws()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().stb_icp_jacobians(
+                ctypes.byref(level.struct), ctypes.byref(params), ws, nbytes,
+                _lib.stream()), "stb_icp_jacobians")
+
     def information(self, level, params):
         ws, nbytes = self._ws()
         with torch.cuda.device(self.device):

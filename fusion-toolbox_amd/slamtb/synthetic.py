@@ -160,14 +160,3 @@ def make_pair(seed, width, height, kcam=None, channels=3, descriptor=False,
     return dict(kcam=np.asarray(kcam, np.float32), src_depth=src_depth,
                 tgt_depth=tgt_depth, src_feats=src_feats, tgt_feats=tgt_feats,
                 T_gt=T_gt, seed=seed, width=width, height=height)
-
-
-def config_kcam(config):
-    """Intrinsics of the 320x240 and 640x480 single-pair workloads."""
-    if config == 1:
-        return np.array([[262.5, 0, 159.5], [0, 262.5, 119.5], [0, 0, 1]],
-                        dtype=np.float32)
-    if config == 2:
-        return np.array([[525.0, 0, 319.5], [0, 525.0, 239.5], [0, 0, 1]],
-                        dtype=np.float32)
-    raise ValueError(config)

@@ -231,6 +231,10 @@ typedef struct stb_icp_state {
   int32_t map_sel;        /* correspondence map the next iteration uses */
 } stb_icp_state;
 
+/* sizeof(stb_icp_state) as compiled into the library (for bindings that
+ * mirror the struct). No counterpart in the reference. */
+size_t stb_icp_state_bytes(void);
+
 /* Bytes of workspace needed for a level of n_src source and height*width
  * target pixels (state, two correspondence maps, block partials). */
 size_t stb_icp_workspace_bytes(int64_t n_src, int32_t height, int32_t width);
@@ -265,6 +269,14 @@ int stb_icp_correspond(const stb_icp_level* level, const stb_icp_params* params,
                        void* workspace, size_t workspace_bytes, void* stream);
 int stb_icp_update(const stb_icp_level* level, const stb_icp_params* params,
                    void* workspace, size_t workspace_bytes, void* stream);
+
+/* The Jacobian launch of one iteration alone, over the correspondence map
+ * of the last completed iteration or stb_icp_linearize call: per-block
+ * partials into the workspace, with no reduction, no pose update and both
+ * maps untouched, so that repeated calls do identical work. For timing the
+ * dominant launch of an iteration on its own. */
+int stb_icp_jacobians(const stb_icp_level* level, const stb_icp_params* params,
+                      void* workspace, size_t workspace_bytes, void* stream);
 
 /* One linearisation at the current pose without a pose update: fills
  * last_jtj / last_jr / last_*_r2 / last_*_count with the reduced system of

@@ -781,6 +781,16 @@ int orc_num_threads(void) {
 #endif
 }
 
+// Thread count of the OpenMP loops (the CPU baseline is timed at 1 thread
+// and at all of the host's threads).
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
 void orc_corresp_map_f32(const float* sp, const float* sn, const uint8_t* sm,
                          int64_t ml, int64_t n, const float* rt,
                          const float* tp, const float* tn, const uint8_t* tm,

@@ -51,6 +51,8 @@ def lib():
         _LIB.orc_feature_f32.restype = ctypes.c_int
         _LIB.orc_feature_f64.restype = ctypes.c_int
         _LIB.orc_num_threads.restype = ctypes.c_int
+        _LIB.orc_set_num_threads.argtypes = [ctypes.c_int]
+        _LIB.orc_set_num_threads.restype = None
     return _LIB
 
 
@@ -80,6 +82,14 @@ def _rt34(rt, dtype):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_num_threads(n):
+    """Thread count of the parallel (OpenMP) oracle loops and of torch's CPU
+    reductions that the parallel variant uses."""
+    import torch
+    lib().orc_set_num_threads(int(n))
+    torch.set_num_threads(int(n))
 
 
 # ---------------------------------------------------------------------------
@@ -336,6 +346,17 @@ def _cholesky_solve(JtJ, Jr):
     return np.linalg.solve(L.T, y)
 
 
+def _sum0(a, sum_dtype, parallel):
+    """Sum over the pixel axis, widened to FP64 for the solve. The parallel
+    (CPU-baseline) variant sums with torch on the CPU in the partials' own
+    dtype, as the reference does (icp.py:78-80); the sequential oracle sums
+    in ``sum_dtype`` with numpy."""
+    if parallel:
+        import torch
+        return torch.from_numpy(np.ascontiguousarray(a)).sum(0).double().numpy()
+    return np.asarray(a.sum(0, dtype=sum_dtype), np.float64)
+
+
 def icp_estimate(kcam, src_pts, src_nrm, src_mask, tgt_pts, tgt_nrm, tgt_mask,
                  num_iters, src_feats=None, tgt_feats=None, transform=None,
                  geom_weight=1.0, feat_weight=1.0, distance_threshold=0.5,
@@ -345,8 +366,9 @@ def icp_estimate(kcam, src_pts, src_nrm, src_mask, tgt_pts, tgt_nrm, tgt_mask,
     """One pyramid level of ``ICPOdometry.estimate`` (icp.py:152-262).
 
     ``dtype`` is the per-pixel arithmetic type; the partials are summed in
-    ``sum_dtype`` (FP64 for the pose oracle, FP32 like torch for the CPU
-    baseline) and the system is solved in FP64.
+    ``sum_dtype`` (FP64 for the pose oracle); with ``parallel`` (the CPU
+    baseline: OpenMP pixel loops) they are summed by torch in FP32 like the
+    reference. The system is solved in FP64.
     """
     dt = np.dtype(dtype)
     T = np.eye(4) if transform is None else np.asarray(transform, np.float64).copy()
@@ -377,9 +399,9 @@ def icp_estimate(kcam, src_pts, src_nrm, src_mask, tgt_pts, tgt_nrm, tgt_mask,
                                parallel=parallel)
             jtj, jr, r2, cnt = feature(sp, sf, sm, rt, tf, kcam, cmap,
                                        feat_residual_thresh, parallel=parallel)
-            JtJ = jtj.sum(0, dtype=sum_dtype).astype(np.float64) * feat_weight * feat_weight
-            Jr = jr.sum(0, dtype=sum_dtype).astype(np.float64) * -feat_weight
-            residual = float(r2.sum(dtype=sum_dtype)) * feat_weight
+            JtJ = _sum0(jtj, sum_dtype, parallel) * feat_weight * feat_weight
+            Jr = _sum0(jr, sum_dtype, parallel) * -feat_weight
+            residual = float(_sum0(r2, sum_dtype, parallel)) * feat_weight
             count = cnt
         if has_geom:
             cmap = corresp_map(sp, sn, sm, rt, tp, tn, tgt_mask, kcam,
@@ -387,9 +409,9 @@ def icp_estimate(kcam, src_pts, src_nrm, src_mask, tgt_pts, tgt_nrm, tgt_mask,
                                parallel=parallel)
             jtj, jr, r2, cnt = geometric(sp, sm, rt, tp32, tn32, kcam, cmap,
                                          parallel=parallel)
-            JtJ = JtJ + jtj.sum(0, dtype=sum_dtype).astype(np.float64) * geom_weight * geom_weight
-            Jr = Jr + jr.sum(0, dtype=sum_dtype).astype(np.float64) * geom_weight
-            residual = residual + float(r2.sum(dtype=sum_dtype)) * geom_weight
+            JtJ = JtJ + _sum0(jtj, sum_dtype, parallel) * geom_weight * geom_weight
+            Jr = Jr + _sum0(jr, sum_dtype, parallel) * geom_weight
+            residual = residual + float(_sum0(r2, sum_dtype, parallel)) * geom_weight
             count = max(cnt, count)
         x = _cholesky_solve(JtJ, Jr)
         if x is None:

@@ -32,8 +32,13 @@ def test_binding_covers_header():
 
 
 def test_state_struct_size_matches_library_layout():
-    # stb_icp_state: 16+16+36+36+1+1+36+6+4 doubles, 12+9 floats, 4 ints.
+    # The ctypes mirror has the size the library was compiled with:
+    # 16+16+36+36+1+1+36+6+4 doubles, 12+9 floats, 4 ints (+4 bytes of tail
+    # padding to the 8-byte alignment).
+    assert ctypes.sizeof(_lib.StbIcpState) == _lib.lib().stb_icp_state_bytes()
     assert ctypes.sizeof(_lib.StbIcpState) == 152 * 8 + 21 * 4 + 4 * 4 + 4
+    assert _lib.StbIcpState.rt.offset == 152 * 8
+    assert _lib.StbIcpState.map_sel.offset == 152 * 8 + 21 * 4 + 3 * 4
 
 
 def test_workspace_size_grows_with_the_target():

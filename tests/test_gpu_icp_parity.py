@@ -30,9 +30,9 @@ def small_motion(seed=11, angle=2.0, trans=0.02):
 
 
 @pytest.fixture(scope="module")
-def cfg1():
+def pair320():
     """320x240 pair of the single-level geometric workload."""
-    k = synthetic.config_kcam(1)
+    k = synthetic.intrinsics(320, 240).astype(np.float32)
     return prepared_pair(3, 320, 240, kcam=k)
 
 
@@ -54,53 +54,53 @@ def cpu_map(P, T, dtype=np.float32, dthr=0.1, athr=ANG20):
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("which", ["identity", "motion"])
-def test_correspondence_bit_exact(cfg1, dtype, which):
+def test_correspondence_bit_exact(pair320, dtype, which):
     T = np.eye(4) if which == "identity" else small_motion()
     npdt = np.float32 if dtype == torch.float32 else np.float64
-    got = gpu_map(cfg1, T, dtype)
-    ref = cpu_map(cfg1, T, npdt)
+    got = gpu_map(pair320, T, dtype)
+    ref = cpu_map(pair320, T, npdt)
     print(f"matched pixels: {(ref != orc.CORRESP_INF).sum()}")
     assert (ref != orc.CORRESP_INF).sum() > 1000
     assert np.array_equal(got, ref)
 
 
-def test_correspondence_thresholds_and_ties(cfg1):
+def test_correspondence_thresholds_and_ties(pair320):
     # A huge distance threshold and a half-turn angle threshold admit many
     # colliding candidates, so min-merge ties and the angle edge both matter.
     T = small_motion(5, 4.0, 0.05)
     for dthr, athr in [(10.0, math.pi), (0.02, 0.1), (1.0, 0.0), (0.1, 3.5)]:
-        assert np.array_equal(gpu_map(cfg1, T, dthr=dthr, athr=athr),
-                              cpu_map(cfg1, T, dthr=dthr, athr=athr)), (dthr, athr)
+        assert np.array_equal(gpu_map(pair320, T, dthr=dthr, athr=athr),
+                              cpu_map(pair320, T, dthr=dthr, athr=athr)), (dthr, athr)
 
 
-def test_correspondence_empty_mask(cfg1):
-    P = dict(cfg1)
-    P["src_mask"] = np.zeros_like(cfg1["src_mask"])
+def test_correspondence_empty_mask(pair320):
+    P = dict(pair320)
+    P["src_mask"] = np.zeros_like(pair320["src_mask"])
     got = gpu_map(P, np.eye(4))
     assert np.all(got == orc.CORRESP_INF)
 
 
-def test_correspondence_runs_twice_identically(cfg1):
+def test_correspondence_runs_twice_identically(pair320):
     T = small_motion()
-    assert np.array_equal(gpu_map(cfg1, T), gpu_map(cfg1, T))
+    assert np.array_equal(gpu_map(pair320, T), gpu_map(pair320, T))
 
 
-def test_geometric_partials(cfg1):
+def test_geometric_partials(pair320):
     T = small_motion()
-    cmap = cpu_map(cfg1, T)
-    g = to_dev(cfg1, DEV)
-    n = cfg1["src_pts"].reshape(-1, 3).shape[0]
+    cmap = cpu_map(pair320, T)
+    g = to_dev(pair320, DEV)
+    n = pair320["src_pts"].reshape(-1, 3).shape[0]
     JtJ = torch.empty(n, 6, 6, device=DEV)
     Jr = torch.empty(n, 6, device=DEV)
     r2 = torch.empty(n, device=DEV)
     cnt = ICPJacobian.estimate_geometric(
         g["src_pts"].view(-1, 3), g["src_mask"].view(-1),
         torch.from_numpy(T).float().to(DEV), g["tgt_pts"], g["tgt_nrm"],
-        torch.from_numpy(cfg1["kcam"]).to(DEV), torch.from_numpy(cmap).to(DEV),
+        torch.from_numpy(pair320["kcam"]).to(DEV), torch.from_numpy(cmap).to(DEV),
         JtJ, Jr, r2)
-    rj, rr, rr2, rc = orc.geometric(cfg1["src_pts"], cfg1["src_mask"],
-                                    T[:3].astype(np.float32), cfg1["tgt_pts"],
-                                    cfg1["tgt_nrm"], cfg1["kcam"], cmap)
+    rj, rr, rr2, rc = orc.geometric(pair320["src_pts"], pair320["src_mask"],
+                                    T[:3].astype(np.float32), pair320["tgt_pts"],
+                                    pair320["tgt_nrm"], pair320["kcam"], cmap)
     assert cnt == rc and rc > 1000
     np.testing.assert_array_max_ulp(JtJ.cpu().numpy(), rj, maxulp=2)
     np.testing.assert_array_max_ulp(Jr.cpu().numpy(), rr, maxulp=2)
@@ -130,19 +130,19 @@ def test_feature_partials():
     np.testing.assert_array_max_ulp(r2.cpu().numpy(), rr2, maxulp=2)
 
 
-def test_information_partials(cfg1):
+def test_information_partials(pair320):
     T = small_motion()
-    cmap = cpu_map(cfg1, T)
-    g = to_dev(cfg1, DEV)
+    cmap = cpu_map(pair320, T)
+    g = to_dev(pair320, DEV)
     n = 320 * 240
     GtG = torch.empty(n, 6, 6, device=DEV)
     ICPJacobian.get_information_matrix(
         g["src_pts"].view(-1, 3), g["src_mask"].view(-1),
         torch.from_numpy(T).float().to(DEV), g["tgt_pts"],
-        torch.from_numpy(cfg1["kcam"]).to(DEV), torch.from_numpy(cmap).to(DEV), GtG)
-    ref = orc.information(cfg1["src_pts"], cfg1["src_mask"],
-                          T[:3].astype(np.float32), cfg1["tgt_pts"],
-                          cfg1["kcam"], cmap)
+        torch.from_numpy(pair320["kcam"]).to(DEV), torch.from_numpy(cmap).to(DEV), GtG)
+    ref = orc.information(pair320["src_pts"], pair320["src_mask"],
+                          T[:3].astype(np.float32), pair320["tgt_pts"],
+                          pair320["kcam"], cmap)
     np.testing.assert_array_max_ulp(GtG.cpu().numpy(), ref, maxulp=2)
 
 
@@ -220,15 +220,15 @@ def run_icp(P, iters, gw=1.0, fw=0.0, dthr=0.1, athr=ANG20, fthr=2.75, T0=None):
                         transform=None if T0 is None else torch.from_numpy(T0))
 
 
-def test_icp_config1_matches_oracle(cfg1):
-    res = run_icp(cfg1, 10)
-    ref = orc.icp_estimate(cfg1["kcam"], cfg1["src_pts"], cfg1["src_nrm"],
-                           cfg1["src_mask"], cfg1["tgt_pts"], cfg1["tgt_nrm"],
-                           cfg1["tgt_mask"], 10, geom_weight=1, feat_weight=0,
+def test_icp_config1_matches_oracle(pair320):
+    res = run_icp(pair320, 10)
+    ref = orc.icp_estimate(pair320["kcam"], pair320["src_pts"], pair320["src_nrm"],
+                           pair320["src_mask"], pair320["tgt_pts"], pair320["tgt_nrm"],
+                           pair320["tgt_mask"], 10, geom_weight=1, feat_weight=0,
                            distance_threshold=0.1, normals_angle_thresh=ANG20)
     err = orc.pose_error(res.transform.numpy(), ref["transform"])
     print("pose error vs oracle", err, "vs ground truth",
-          orc.pose_error(res.transform.numpy(), cfg1["T_gt"]))
+          orc.pose_error(res.transform.numpy(), pair320["T_gt"]))
     assert err[0] < 1e-6 and err[1] < 1e-6
     assert res.iterations == ref["iters"] == 10
     assert abs(res.match_ratio - ref["match_ratio"]) < 1e-3
@@ -249,9 +249,9 @@ def test_icp_features_matches_oracle():
     assert err[0] < 1e-6 and err[1] < 1e-6
 
 
-def test_icp_zero_overlap_keeps_no_transform(cfg1):
+def test_icp_zero_overlap_keeps_no_transform(pair320):
     far = np.eye(4)
     far[0, 3] = 100.0  # every source point leaves the image
-    res = run_icp(cfg1, 5, T0=far)
+    res = run_icp(pair320, 5, T0=far)
     assert res.transform is None and res.iterations == 0
     assert math.isinf(res.residual)

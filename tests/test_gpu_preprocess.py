@@ -29,7 +29,7 @@ DEV = torch.device("cuda:0")
 
 @pytest.fixture(scope="module")
 def pair():
-    return synthetic.make_pair(21, 320, 240, kcam=synthetic.config_kcam(1))
+    return synthetic.make_pair(21, 320, 240, kcam=synthetic.intrinsics(320, 240).astype(np.float32))
 
 
 def g(a):
@@ -50,7 +50,7 @@ def test_bilateral_float_and_int(pair):
 
 def test_erode_normals_points_exact(pair):
     d = pair["src_depth"]
-    k = synthetic.config_kcam(1)
+    k = synthetic.intrinsics(320, 240).astype(np.float32)
     m = processing.erode_mask(g(d > 0))
     assert np.array_equal(m.cpu().numpy(), orc.erode(d > 0))
     frame = Frame(FrameInfo(KCamera(torch.from_numpy(k))), depth_image=g(d))
@@ -62,7 +62,7 @@ def test_erode_normals_points_exact(pair):
 
 def test_pyramid_and_feature_pyramid_exact(pair):
     d = pair["src_depth"]
-    k = synthetic.config_kcam(1)
+    k = synthetic.intrinsics(320, 240).astype(np.float32)
     frame = Frame(FrameInfo(KCamera(torch.from_numpy(k))), depth_image=g(d))
     fp = FramePointCloud.from_frame(frame, device=DEV)
     pyr = fp.pyramid([1.0, 0.5, 0.5])
