@@ -28,7 +28,10 @@ __global__ void __launch_bounds__(kBlock)
                          unsigned long long* __restrict__ map0,
                          unsigned long long* __restrict__ map1, float dthr2,
                          float cstar, int fixed_map) {
-  if (!fixed_map && st->stopped) return;
+  // No early exit on state.stopped: a stopped loop's remaining launches
+  // only rewrite maps that nothing reads again (the solve launch checks the
+  // flag and leaves the state alone), and not testing it first lets the
+  // pose, the map choice and the pixel data load together.
   unsigned long long* __restrict__ map = pick_map(st, map0, map1, fixed_map);
   float m[12], nm[9];
 #pragma unroll
@@ -43,7 +46,11 @@ __global__ void __launch_bounds__(kBlock)
 
 // Per-pixel Jacobians of one iteration; every block leaves one FP64 partial
 // for solve_fused_kernel (the next launch), so no block waits on another.
-template <typename S, bool GEOM, bool FEAT>
+// With ONE_PX each thread owns exactly one pixel (levels that fit the grid
+// cap): the slot accumulators are then born from that pixel's terms instead
+// of living across a loop (~84 VGPRs instead of ~116). Larger levels use the
+// grid-stride loop.
+template <typename S, bool GEOM, bool FEAT, bool ONE_PX>
 __global__ void __launch_bounds__(kBlock)
     jacobian_fused_kernel(stb_icp_level Ld, stb_icp_state* __restrict__ st,
                           unsigned long long* __restrict__ map0,
@@ -51,15 +58,12 @@ __global__ void __launch_bounds__(kBlock)
                           int64_t map_len, double* __restrict__ partials,
                           TermWeights tw, int fixed_map) {
   __shared__ double sh[kShared];
-  if (!fixed_map && st->stopped) return;
+  // No early exit on state.stopped (see corresp_fused_kernel): the solve
+  // launch ignores the partials of a stopped loop.
   const unsigned long long* __restrict__ map =
       pick_map(st, map0, map1, fixed_map);
   const int64_t gsize = int64_t(gridDim.x) * blockDim.x;
   const int64_t gtid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (!fixed_map) {  // clear the other map for the next iteration
-    unsigned long long* __restrict__ next = pick_map(st, map0, map1, 1);
-    for (int64_t i = gtid; i < map_len; i += gsize) next[i] = kKeySentinel;
-  }
   float m[12];
 #pragma unroll
   for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
@@ -67,8 +71,16 @@ __global__ void __launch_bounds__(kBlock)
   float acc[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
-  for (int64_t idx = gtid; idx < L.n; idx += gsize)
-    jacobian_pixel<S, GEOM, FEAT>(L, m, map, idx, tw, acc);
+  if (ONE_PX) {
+    if (gtid < L.n) jacobian_pixel<S, GEOM, FEAT>(L, m, map, gtid, tw, acc);
+  } else {
+    for (int64_t idx = gtid; idx < L.n; idx += gsize)
+      jacobian_pixel<S, GEOM, FEAT>(L, m, map, idx, tw, acc);
+  }
+  if (!fixed_map) {  // clear the other map for the next iteration
+    unsigned long long* __restrict__ next = pick_map(st, map0, map1, 1);
+    for (int64_t i = gtid; i < map_len; i += gsize) next[i] = kKeySentinel;
+  }
   block_partial(acc, partials + int64_t(blockIdx.x) * kSlots, sh, false);
 }
 
@@ -158,7 +170,8 @@ __global__ void __launch_bounds__(kBlock)
 // ---------------------------------------------------------------------------
 
 constexpr int kDefaultUpdateBlocks = 1024;
-constexpr int kMaxUpdateBlocks = 2048;
+constexpr int kMaxUpdateBlocks = 4096;
+constexpr int kInformationBlocks = 256;
 constexpr int kSolveGroups = 16;
 constexpr int kDefaultCorrespBlocks = 4096;
 
@@ -193,10 +206,16 @@ Workspace carve(void* base, int32_t H, int32_t W) {
   return w;
 }
 
-int update_grid(const stb_icp_params& P, int64_t n) {
+// Jacobian grid: one pixel per thread when that fits the cap (grid_update,
+// or kDefaultUpdateBlocks), else `cap` blocks looping over pixels. The
+// kernel is bound by vector-instruction issue once all its loads are in
+// flight, and at 640x480 1024 looping blocks beat 1200 one-pixel blocks
+// (7.3 against 7.9 us): each block's reduction is shared by more pixels.
+int update_grid(const stb_icp_params& P, int64_t n, bool* one_px) {
   int cap = P.grid_update > 0 ? P.grid_update : kDefaultUpdateBlocks;
   if (cap > kMaxUpdateBlocks) cap = kMaxUpdateBlocks;
   const int g = grid_for(n);
+  *one_px = g <= cap;
   return g > cap ? cap : g;
 }
 
@@ -232,10 +251,16 @@ template <typename S, bool GEOM, bool FEAT>
 void launch_jacobian(const stb_icp_level& L, const stb_icp_params& P,
                      const Workspace& w, const TermWeights& tw, int do_solve,
                      int fixed_map, hipStream_t stream) {
-  const int grid = update_grid(P, L.n_src);
-  hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT>), dim3(grid),
-                     dim3(kBlock), 0, stream, L, w.st, w.map[0], w.map[1],
-                     w.map_len, w.partials, tw, fixed_map);
+  bool one_px = false;
+  const int grid = update_grid(P, L.n_src, &one_px);
+  if (one_px)
+    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, true>), dim3(grid),
+                       dim3(kBlock), 0, stream, L, w.st, w.map[0], w.map[1],
+                       w.map_len, w.partials, tw, fixed_map);
+  else
+    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, false>),
+                       dim3(grid), dim3(kBlock), 0, stream, L, w.st, w.map[0],
+                       w.map[1], w.map_len, w.partials, tw, fixed_map);
   if (do_solve < 0) return;  // the Jacobian launch alone
   const int groups = grid < kSolveGroups ? grid : kSolveGroups;
   hipLaunchKernelGGL(solve_fused_kernel, dim3(groups), dim3(kBlock), 0, stream,
@@ -263,9 +288,12 @@ void launch_phase(const stb_icp_level& L, const stb_icp_params& P,
   }
   if (phase == kCorrespond) return;
   if (phase == kInformation) {
+    // A one-off pass reduced by its last block: every block pays a fence and
+    // a ticket, so a few hundred blocks loop over the pixels.
     hipLaunchKernelGGL(information_fused_kernel<S>,
-                       dim3(update_grid(P, L.n_src)), dim3(kBlock), 0, stream,
-                       L, w.st, w.map[0], w.map[1], w.partials);
+                       dim3(grid_for(L.n_src, kBlock, kInformationBlocks)),
+                       dim3(kBlock), 0, stream, L, w.st, w.map[0], w.map[1],
+                       w.partials);
     return;
   }
   const TermWeights tw = make_weights(P);

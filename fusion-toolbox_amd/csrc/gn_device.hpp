@@ -185,9 +185,8 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
   }
   if (FEAT) {
     float jf[6], rl;
-    const float sq = feature_terms<float, S, S>(L.tf, L.C, L.H, L.W, L.HW, L.sf,
-                                                L.n, idx, L.fx, L.fy, p, u, v,
-                                                jf, rl);
+    const float sq = feature_terms<float, S, S, true>(
+        L.tf, L.C, L.H, L.W, L.HW, L.sf, L.n, idx, L.fx, L.fy, p, u, v, jf, rl);
     const bool use = own && !(sq > tw.fthr2);
     float j[6];
 #pragma unroll

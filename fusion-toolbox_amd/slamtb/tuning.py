@@ -12,8 +12,8 @@ import torch
 
 from slamtb.registration._fused import runner_for
 
-UPDATE_CANDIDATES = (256, 512, 768, 1024, 1536, 2048)
-CORRESP_CANDIDATES = (1024, 2048, 4096, 8192)
+UPDATE_CANDIDATES = (0, 512, 2048, 4096)  # 0: the library default (1024)
+CORRESP_CANDIDATES = (0, 1024, 2048)  # 0: one pixel per thread up to 4096 blocks
 TILE_CANDIDATES = (256, 512, 1024, 2048)
 
 _TABLE = {}
