@@ -31,6 +31,8 @@ synchronisations. Also reported:
   * 1280x720 with 32-channel descriptors in BF16 storage against FP32;
   * 5 unrolled differentiable GN steps on 8 pairs at 640x480, forward and
     adjoint backward;
+  * the differentiable frame matcher and projection kernels at 640x480 and
+    AutogradICP driven by them, with the oracle's matcher beside them;
   * the CPU baseline: the oracle's OpenMP port of the reference CPU path on
     this host, at all threads and at one thread.
 """
@@ -67,6 +69,8 @@ def parse():
                     help="pairs of the batched workload (0 disables it)")
     ap.add_argument("--no-bf16", action="store_true",
                     help="skip the 1280x720 BF16-storage workload")
+    ap.add_argument("--no-matcher", action="store_true",
+                    help="skip the differentiable matcher / AutogradICP section")
     ap.add_argument("--no-diff", action="store_true",
                     help="skip the differentiable 8 x 640x480 workload")
     ap.add_argument("--no-tune", action="store_true",
@@ -477,6 +481,127 @@ def diff_section(seed, dev, batch=8, steps=5):
                                     for t in leaves + [x0]))}
 
 
+def matcher_section(pair, top, dev, with_cpu):
+    """The differentiable frame matcher at 640x480 (every valid source pixel,
+    moved by the true motion, matched against the target frame with its
+    3-channel features) and AutogradICP driven by it: per-launch durations
+    of the matcher and projection kernels, the whole autograd forward +
+    backward, and the oracle's single-thread matcher on the same inputs."""
+    import numpy as np
+    import torch
+    from slamtb._cslamtb import FPCLMatcherOp as Op, ProjectOp
+    from slamtb.camera import KCamera
+    from slamtb.registration.autogradicp import AutogradICP
+    from slamtb.registration.correspondence_map import CorrespondenceMap
+    from slamtb.spatial.matching import FPCLMatcherOp, FramePointCloudMatcher
+
+    k = torch.tensor([[top.fx, 0, top.cx], [0, top.fy, top.cy], [0, 0, 1]],
+                     dtype=torch.float32)
+    kd = k.to(dev)
+    T = torch.from_numpy(pair["T_gt"]).to(dev, torch.float32)
+    m = top.src_mask.view(-1).bool()
+    sp = (top.src_points.view(-1, 3)[m].float() @ T[:3, :3].T + T[:3, 3]).contiguous()
+    sn = (top.src_normals.view(-1, 3)[m].float() @ T[:3, :3].T).contiguous()
+    tp, tn = top.tgt_points.float(), top.tgt_normals.float()
+    tf = top.tgt_feats.float().contiguous()
+    C, H, W = tf.shape
+    n = sp.size(0)
+    ones = torch.ones(1, dtype=torch.bool, device=dev)
+    eye = torch.eye(4, device=dev)
+    cm = CorrespondenceMap(0.1, math.radians(20))
+    cmap = cm(sp, sn, ones, eye, tp, tn, top.tgt_mask, kd)
+    op = torch.empty(n, 3, device=dev)
+    on = torch.empty(n, 3, device=dev)
+    of = torch.empty(C, n, device=dev)
+    mask = torch.empty(n, dtype=torch.bool, device=dev)
+    dl = torch.randn(C, n, device=dev)
+    dx = torch.empty(n, 3, device=dev)
+    fwd = lambda: Op.forward(tp, tn, tf, sp, kd, cmap, op, on, of, mask)
+    bwd = lambda: Op.backward(tf, sp, mask, dl, kd, 0.005, dx)
+    fwd()
+    matched = float(mask.float().mean().item())
+    t_map = launch_ms(lambda: cm(sp, sn, ones, eye, tp, tn, top.tgt_mask, kd), 50)
+    t_fwd = launch_ms(fwd, 100)
+    t_bwd = launch_ms(bwd, 100)
+    dy = torch.randn(n, 2, device=dev)
+    t_pf = launch_ms(lambda: ProjectOp.forward(sp, kd), 100)
+    t_pb = launch_ms(lambda: ProjectOp.backward(dy, sp, kd), 100)
+    # Algorithmic bytes: forward reads the point (12) and its map word (8);
+    # a matched point gathers the target point and normal (24) and one texel
+    # per channel (neighbouring taps are shared between neighbouring points),
+    # and every point writes its rows (24 + 4C) and mask byte. Backward reads
+    # the mask, and for matched points the point (12), incoming gradient (4C)
+    # and one texel per channel; it writes 12.
+    b_fwd = n * (12 + 8 + 1 + 24 + 4 * C) + n * matched * (24 + 4 * C)
+    b_bwd = n * (1 + 12) + n * matched * (12 + 8 * C)
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
+
+    kc = KCamera(k)
+    matcher = FramePointCloudMatcher(tp, tn, top.tgt_mask, tf, kc, 0.1,
+                                     math.radians(20))
+    src = sp.clone().requires_grad_(True)
+
+    def autograd_step():
+        src.grad = None
+        pts, nrm, feats, msk = matcher.find_correspondences(src, sn)
+        loss = ((pts - src[msk]) * nrm).sum(1).pow(2).mean() + feats.pow(2).mean()
+        loss.backward()
+    autograd_step()
+    torch.cuda.synchronize()
+    walls = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        autograd_step()
+        torch.cuda.synchronize()
+        walls.append((time.perf_counter() - t0) * 1e3)
+
+    # AutogradICP from the identity, counting cost evaluations.
+    calls = [0]
+
+    class Counting(FramePointCloudMatcher):
+        def find_correspondences(self, *a):
+            calls[0] += 1
+            return super().find_correspondences(*a)
+    sm = top.src_mask.bool()
+    src_pts = top.src_points.float()[sm].view(-1, 3)
+    src_nrm = top.src_normals.float()[sm].view(-1, 3)
+    src_f = top.src_feats.float()[:, sm].contiguous()
+    icp = AutogradICP(20, learning_rate=0.1, geom_weight=0.5, feat_weight=0.5,
+                      distance_threshold=0.1, normals_angle_thresh=math.radians(20),
+                      feat_residual_thresh=2.75, gradient_descent_iters=20)
+    cnt = Counting(tp, tn, top.tgt_mask, tf, kc, 0.1, math.radians(20))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = icp._estimate(src_pts, src_nrm, src_f, cnt)
+    el = time.perf_counter() - t0
+    from slamtb import metrics
+    out = {"points": n, "matched_frac": matched, "channels": C,
+           "launch_ms": {"correspondence_map": t_map, "fpcl_forward_kernel": t_fwd,
+                         "fpcl_backward_kernel": t_bwd,
+                         "project_forward_kernel": t_pf,
+                         "project_backward_kernel": t_pb},
+           "fpcl_forward_gbs": gbs(b_fwd, t_fwd), "fpcl_backward_gbs": gbs(b_bwd, t_bwd),
+           "fpcl_forward_bytes": b_fwd, "fpcl_backward_bytes": b_bwd,
+           "autograd_match_forward_backward_ms": statistics.median(walls),
+           "autogradicp": {"evaluations": calls[0], "seconds": el,
+                           "ms_per_evaluation": el * 1e3 / max(1, calls[0]),
+                           "pose_err_vs_truth": [float(e) for e in metrics.pose_error(
+                               res.transform.double().numpy(), pair["T_gt"])]}}
+    if with_cpu:
+        sys.path.insert(0, str(ROOT))
+        from oracle import oracle as orc
+        c = lambda x: x.detach().cpu().numpy()
+        a = [c(tp), c(tn), c(tf), c(sp), c(kd), c(cmap)]
+        t0 = time.perf_counter()
+        r = orc.fpcl_forward(*a)
+        t1 = time.perf_counter()
+        orc.fpcl_backward(a[2], a[3], r[3], c(dl), a[4], 0.005)
+        t2 = time.perf_counter()
+        out["cpu_oracle_1thread_ms"] = {"fpcl_forward": (t1 - t0) * 1e3,
+                                        "fpcl_backward": (t2 - t1) * 1e3}
+    return out
+
+
 def main():
     args = parse()
     if args.copy_probe > 0:
@@ -491,7 +616,7 @@ def main():
     from slamtb.registration._fused import runner_for
     if args.profile:
         args.no_oracle = args.no_cpu_baseline = args.no_bf16 = True
-        args.no_diff = True
+        args.no_diff = args.no_matcher = True
         args.pairs = 0
 
     dev = torch.device("cuda:0")
@@ -607,6 +732,9 @@ def main():
     if not args.no_bf16:
         log("1280x720 BF16 / FP32 storage workload")
         out["bf16_1280x720"] = bf16_section(args.seed, dev, not args.no_oracle)
+    if not args.no_matcher:
+        log("differentiable matcher and AutogradICP at 640x480")
+        out["matcher_640x480"] = matcher_section(pair, top, dev, not args.no_oracle)
     if not args.no_diff:
         log("differentiable GN: 8 pairs at 640x480, 5 steps, forward + backward")
         out["diff_8x640x480"] = diff_section(args.seed, dev)

@@ -66,7 +66,7 @@ __device__ inline Bil<T> make_bil(T u, T v) {
 // edge re-reads an in-range texel and is then replaced by zero), so they
 // issue together with no branches between them. The cell (ui, vi) is in
 // range for every caller: u and v lie in [-0.5, W - 0.5) x [-0.5, H - 0.5)
-// (plus at most h = 0.05), which truncates into the image.
+// (plus at most h < 0.5), which truncates into the image.
 template <typename T, typename S>
 __device__ inline void bil_taps(const S* __restrict__ ch, int H, int W,
                                 const Bil<T>& b, T t[4]) {
@@ -92,11 +92,7 @@ __device__ inline T bil_mix(int H, int W, const Bil<T>& b, const T t[4]) {
   return a * (T(1) - b.vr) + c * b.vr;
 }
 
-// The four taps are loaded unconditionally (a tap past the right or bottom
-// edge re-reads an in-range texel and is then replaced by zero), so they
-// issue together with no branches between them. The cell (ui, vi) is in
-// range for every caller: u and v lie in [-0.5, W - 0.5) x [-0.5, H - 0.5)
-// (plus at most h = 0.05), which truncates into the image.
+// Bilinear sample of one channel (src/feature_map.hpp:29-42).
 template <typename T, typename S>
 __device__ inline T bil_get(const S* __restrict__ ch, int H, int W,
                             const Bil<T>& b) {

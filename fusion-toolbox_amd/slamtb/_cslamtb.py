@@ -9,6 +9,8 @@ output convention as the pybind11 bindings:
 * ``Processing``, ``EstimateNormalsMethod``, ``DownsampleXYZMethod`` --
   src/processing.cpp:7-22
 * ``ExpRtToMatrixOp``, ``MatrixToExpRtOp`` -- src/se3.cpp:6-15
+* ``FPCLMatcherOp`` -- src/matching.cpp:8-9
+* ``ProjectOp`` -- src/camera.cpp:11-12
 
 Differences, all stricter or documented: tensors must live on a HIP device
 (the reference also had a CPU instantiation; this build has no CPU path),
@@ -332,6 +334,113 @@ class MatrixToExpRtOp:
                 dtype_code(m), ptr(m), m.size(0), ptr(exp_rt), stream()), what)
 
 
+class FPCLMatcherOp:
+    @staticmethod
+    def forward(target_points, target_normals, target_features, source_points,
+                kcam, merge_map, out_points, out_normals, out_features,
+                match_mask):
+        """src/fpcl_matcher.cu:85-123. For each source point: the target pixel
+        it projects onto (rounded) must hold its own index in ``merge_map``;
+        then ``match_mask`` is set and the target point, normal and the
+        bilinear feature sample at the unrounded projection are written.
+        Unmatched rows of the outputs are written as zeros (the reference
+        leaves them unwritten)."""
+        what = "FPCLMatcherOp.forward"
+        require_gpu(target_points, target_normals, target_features,
+                    source_points, kcam, merge_map, out_points, out_normals,
+                    out_features, match_mask, what=what)
+        require_contiguous_out(out_points, out_normals, out_features,
+                               match_mask, what=what)
+        dt = target_points.dtype
+        tp = _in(target_points, dt)
+        tn = _in(target_normals, dt)
+        tf = _in(target_features, dt)
+        C, H, W = tf.shape
+        if tuple(tp.shape[:2]) != (H, W) or tuple(tn.shape[:2]) != (H, W):
+            raise RuntimeError(f"{what}: target points/normals must be [{H}, {W}, 3]")
+        sp = _in(source_points, dt).view(-1, 3)
+        n = sp.size(0)
+        cmap = _in(merge_map)
+        if cmap.dtype != torch.int64 or tuple(cmap.shape) != (H, W):
+            raise RuntimeError(f"{what}: merge_map must be int64 [{H}, {W}]")
+        for t, shape in ((out_points, (n, 3)), (out_normals, (n, 3)),
+                         (out_features, (C, n))):
+            if tuple(t.shape) != shape or t.dtype != dt:
+                raise RuntimeError(f"{what}: expected {shape} {dt} output")
+        if tuple(match_mask.shape) != (n,) or match_mask.dtype not in (
+                torch.bool, torch.uint8):
+            raise RuntimeError(f"{what}: match_mask must be bool [{n}]")
+        with torch.cuda.device(sp.device):
+            check(lib().stb_fpcl_match_forward(
+                dtype_code(sp), ptr(tp), ptr(tn), ptr(tf), C, H, W, ptr(sp), n,
+                ptr(_kcam(kcam, dt)), ptr(cmap), ptr(out_points),
+                ptr(out_normals), ptr(out_features), ptr(match_mask),
+                stream()), what)
+
+    @staticmethod
+    def backward(target_features, source_points, match_mask, dl_feature, kcam,
+                 grad_precision, dx_points):
+        """src/fpcl_matcher.cu:187-229: d loss / d source point through the
+        central-difference gradient of the bilinear feature sample (step
+        ``grad_precision`` pixels) and the projection Jacobian; zero rows
+        where ``match_mask`` is false."""
+        what = "FPCLMatcherOp.backward"
+        require_gpu(target_features, source_points, match_mask, dl_feature,
+                    kcam, dx_points, what=what)
+        require_contiguous_out(dx_points, what=what)
+        dt = target_features.dtype
+        tf = _in(target_features, dt)
+        C, H, W = tf.shape
+        sp = _in(source_points, dt).view(-1, 3)
+        n = sp.size(0)
+        dl = _in(dl_feature, dt)
+        if tuple(dl.shape) != (C, n):
+            raise RuntimeError(f"{what}: dl_feature must be [{C}, {n}]")
+        mm = _mask(match_mask).view(-1)
+        if mm.numel() != n:
+            raise RuntimeError(f"{what}: match_mask must have {n} elements")
+        if tuple(dx_points.shape) != (n, 3) or dx_points.dtype != dt:
+            raise RuntimeError(f"{what}: expected ({n}, 3) {dt} output")
+        with torch.cuda.device(sp.device):
+            check(lib().stb_fpcl_match_backward(
+                dtype_code(sp), ptr(tf), C, H, W, ptr(sp), n, ptr(mm), ptr(dl),
+                ptr(_kcam(kcam, dt)), float(grad_precision), ptr(dx_points),
+                stream()), what)
+
+
+class ProjectOp:
+    @staticmethod
+    def forward(points, intrinsics):
+        """src/camera.cu:34-60: [..., 3] points -> new [N, 2] (u, v) with
+        u = fx*x/z + cx, v = fy*y/z + cy."""
+        what = "ProjectOp.forward"
+        require_gpu(points, intrinsics, what=what)
+        p = _in(points).view(-1, 3)
+        out = torch.empty(p.size(0), 2, dtype=p.dtype, device=p.device)
+        with torch.cuda.device(p.device):
+            check(lib().stb_project_forward(
+                dtype_code(p), ptr(p), p.size(0), ptr(_kcam(intrinsics, p.dtype)),
+                ptr(out), stream()), what)
+        return out
+
+    @staticmethod
+    def backward(dy_grad, points, intrinsics):
+        """src/camera.cu:91-117: new [N, 3] gradient through the projection
+        Jacobian (include/slamtb/camera.hpp:65-78)."""
+        what = "ProjectOp.backward"
+        require_gpu(dy_grad, points, intrinsics, what=what)
+        p = _in(points).view(-1, 3)
+        dy = _in(dy_grad, p.dtype).view(-1, 2)
+        if dy.size(0) != p.size(0):
+            raise RuntimeError(f"{what}: dy_grad must be [{p.size(0)}, 2]")
+        out = torch.empty_like(p)
+        with torch.cuda.device(p.device):
+            check(lib().stb_project_backward(
+                dtype_code(p), ptr(dy), ptr(p), p.size(0),
+                ptr(_kcam(intrinsics, p.dtype)), ptr(out), stream()), what)
+        return out
+
+
 __all__ = ["CorrespondenceMap", "ICPJacobian", "Processing",
            "EstimateNormalsMethod", "DownsampleXYZMethod", "ExpRtToMatrixOp",
-           "MatrixToExpRtOp", "_lib"]
+           "MatrixToExpRtOp", "FPCLMatcherOp", "ProjectOp", "_lib"]

@@ -119,6 +119,13 @@ _SIGNATURES = {
         ctypes.c_int, _vp, _vp, _vp, _i64, _vp, _vp]),
     "stb_matrix_to_exp_rt_forward": (ctypes.c_int, [
         ctypes.c_int, _vp, _i64, _vp, _vp]),
+    "stb_fpcl_match_forward": (ctypes.c_int, [
+        _i32, _vp, _vp, _vp, _i32, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _vp,
+        _vp, _vp, _vp]),
+    "stb_fpcl_match_backward": (ctypes.c_int, [
+        _i32, _vp, _i32, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _f64, _vp, _vp]),
+    "stb_project_forward": (ctypes.c_int, [_i32, _vp, _i64, _vp, _vp, _vp]),
+    "stb_project_backward": (ctypes.c_int, [_i32, _vp, _vp, _i64, _vp, _vp, _vp]),
     "stb_icp_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "stb_icp_begin": (ctypes.c_int, [_vp, _sz, _vp, _i32, _i64, _i32, _i32, _vp]),
     "stb_icp_iterate": (ctypes.c_int, [

@@ -1,8 +1,10 @@
-"""Pinhole intrinsics (the KCamera part of slamtb/camera.py:19-277)."""
+"""Pinhole intrinsics, the differentiable projection and rigid transforms
+(slamtb/camera.py:19-403)."""
 import math
 
 import torch
 
+from ._cslamtb import ProjectOp as _ProjectOp
 from ._utils import ensure_torch
 
 
@@ -79,3 +81,62 @@ class KCamera:
 
     def __str__(self):
         return f"KCamera({self.matrix.tolist()})"
+
+
+class Project(torch.autograd.Function):
+    """Differentiable pinhole projection (camera.py:280-308): [N, 3] camera
+    points and a [3, 3] intrinsic matrix -> [N, 2] image points, both
+    passes on the HIP device."""
+
+    @staticmethod
+    def forward(ctx, points, intrinsics):
+        ctx.save_for_backward(points, intrinsics)
+        return _ProjectOp.forward(points, intrinsics)
+
+    @staticmethod
+    def backward(ctx, dy_grad):
+        points, intrinsics = ctx.saved_tensors
+        return _ProjectOp.backward(dy_grad, points, intrinsics), None
+
+
+class RigidTransform:
+    """A [4, 4] or [3, 4] rigid matrix applied to [N, 3] points with ``@``
+    (camera.py:311-403)."""
+
+    def __init__(self, matrix):
+        self.matrix = matrix
+        self._normal_matrix = None
+
+    def __matmul__(self, points):
+        points = self.matrix[:3, :3].matmul(points.view(-1, 3, 1))
+        points += self.matrix[:3, 3].view(3, 1)
+        return points.squeeze()
+
+    def transform_normals(self, normals):
+        """Normals through the inverse transpose of the rotation block."""
+        return (self.normal_matrix @ normals.view(-1, 3, 1)).view(-1, 3)
+
+    def inplace(self, points):
+        points = points.view(-1, 3)
+        points.copy_(self @ points)
+        return points
+
+    def inplace_normals(self, normals):
+        normals = normals.view(-1, 3)
+        normals.copy_(self.transform_normals(normals))
+        return normals
+
+    @property
+    def normal_matrix(self):
+        if self._normal_matrix is None:
+            self._normal_matrix = normal_transform_matrix(self.matrix)
+        return self._normal_matrix
+
+    def translation(self):
+        return self.matrix[:3, 3]
+
+
+def normal_transform_matrix(matrix):
+    """Transpose of the inverse of the [3, 3] rotation block
+    (camera.py:392-403)."""
+    return torch.inverse(matrix[:3, :3]).transpose(1, 0)

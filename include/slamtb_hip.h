@@ -168,6 +168,45 @@ int stb_exp_rt_to_matrix_backward(int dtype, const void* d_matrix,
 int stb_matrix_to_exp_rt_forward(int dtype, const void* matrix, int64_t batch,
                                  void* exp_rt, void* stream);
 
+/* FPCLMatcherOp.forward (src/matching.cpp:8, src/fpcl_matcher.cu:
+ * 85-123): for each of n_src source points, projected with kcam, the target
+ * pixel that corresp_map (reference layout, [height, width] int64) assigns to
+ * that point gives out_points/out_normals [n_src, 3] and the bilinear
+ * feature sample out_feats [channels, n_src]; match_mask [n_src] (bytes)
+ * flags the matched points. Unmatched rows are written as zeros (the
+ * reference leaves them unwritten). */
+int stb_fpcl_match_forward(int32_t dtype, const void* tgt_points,
+                           const void* tgt_normals, const void* tgt_feats,
+                           int32_t channels, int32_t height, int32_t width,
+                           const void* src_points, int64_t n_src,
+                           const void* kcam, const int64_t* corresp_map,
+                           void* out_points, void* out_normals,
+                           void* out_feats, uint8_t* match_mask, void* stream);
+
+/* FPCLMatcherOp.backward (src/matching.cpp:9, src/fpcl_matcher.cu:
+ * 187-229): dx_points [n_src, 3] = Dx_Project^T (sum_c dI_c/du dl_c,
+ * sum_c dI_c/dv dl_c) for matched points, 0 otherwise; the feature
+ * derivatives are central differences of step grad_precision, which must
+ * lie in (0, 0.5) so that every shifted sample stays inside the image; a
+ * flagged point whose rounded projection is outside the image gets 0. */
+int stb_fpcl_match_backward(int32_t dtype, const void* tgt_feats,
+                            int32_t channels, int32_t height, int32_t width,
+                            const void* src_points, int64_t n_src,
+                            const uint8_t* match_mask, const void* dl_feats,
+                            const void* kcam, double grad_precision,
+                            void* dx_points, void* stream);
+
+/* ProjectOp.forward (src/camera.cpp:11, src/camera.cu:34-60): points [n, 3]
+ * -> uv [n, 2] with u = fx x / z + cx, v = fy y / z + cy; `intrinsics` is a
+ * row-major 3x3 or 2x3 matrix. */
+int stb_project_forward(int32_t dtype, const void* points, int64_t n,
+                        const void* intrinsics, void* out_uv, void* stream);
+
+/* ProjectOp.backward (src/camera.cpp:12, src/camera.cu:89-117). */
+int stb_project_backward(int32_t dtype, const void* dy_uv, const void* points,
+                         int64_t n, const void* intrinsics, void* dx_points,
+                         void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Fused Gauss-Newton path used by slamtb.registration.ICPOdometry.          */
 /* It replaces the per-iteration sequence of icp.py:210-256 (two maps, two   */

@@ -30,7 +30,11 @@
 //   bilateral, normals, erode, downsample: pinned by closed forms on hand
 //     inputs (constant depth, planes, spheres, hand bitmaps);
 //   gaussian blur 3x3: pinned only against the normalised Gaussian formula;
-//     the border rule of the reference's kornia call is parity unpinned.
+//     the border rule of the reference's kornia call is parity unpinned;
+//   fpcl_forward / fpcl_backward / project_*: pinned by closed forms
+//     (features linear in u and v, whose bilinear sample and central
+//     difference are exact), a brute-force matcher over the map, and finite
+//     differences of the projection (test_oracle_matching.py).
 
 #include <cmath>
 #include <cstdint>
@@ -374,6 +378,104 @@ int feature(const T* src_pts, const T* src_feats, const uint8_t* src_mask,
     for (int64_t i = 0; i < n; ++i) count += body(i);
   }
   return int(count);
+}
+
+// ForwardKernel::operator() of FPCLMatcherOp (src/fpcl_matcher.cu:49-83):
+// the rounded projection must land on a target pixel that the map assigns to
+// this very source index; then the target point and normal at that pixel and
+// the bilinear feature sample at the unrounded projection are copied out.
+template <typename T>
+void fpcl_forward(const T* tp, const T* tn, const T* tf, int C, int H, int W,
+                  const T* sp, int64_t n, const T* kcam, const int64_t* map,
+                  T* op, T* on, T* of, uint8_t* mask) {
+  const Cam<T> cam(kcam);
+  const int64_t HW = int64_t(H) * W;
+  for (int64_t i = 0; i < n; ++i) {
+    mask[i] = 0;
+    T u, v;
+    cam.project(load3(sp, i), u, v);
+    const int ui = round_to_int(u), vi = round_to_int(v);
+    if (ui < 0 || ui >= W || vi < 0 || vi >= H) continue;
+    const int64_t pos = int64_t(vi) * W + ui;
+    if (!owns(map, pos, i)) continue;
+    mask[i] = 1;
+    for (int k = 0; k < 3; ++k) {
+      op[3 * i + k] = tp[3 * pos + k];
+      on[3 * i + k] = tn[3 * pos + k];
+    }
+    const Bilinear<T> b(u, v);
+    for (int c = 0; c < C; ++c) of[c * n + i] = b.get(tf + c * HW, H, W);
+  }
+}
+
+// BackwardKernel::operator() of FPCLMatcherOp (src/fpcl_matcher.cu:155-184):
+// the feature gradient in (u, v) by central differences of step h
+// (BilinearInterpGrad, src/feature_map.hpp:45-64), contracted with the
+// incoming gradient over channels, then through Dx_Project
+// (include/slamtb/camera.hpp:65-78).
+template <typename T>
+void fpcl_backward(const T* tf, int C, int H, int W, const T* sp, int64_t n,
+                   const uint8_t* mask, const T* dl, const T* kcam, T h,
+                   T* dx) {
+  const Cam<T> cam(kcam);
+  const int64_t HW = int64_t(H) * W;
+  const T div = T(1) / (T(2) * h);
+  for (int64_t i = 0; i < n; ++i) {
+    dx[3 * i] = dx[3 * i + 1] = dx[3 * i + 2] = T(0);
+    if (!mask[i]) continue;
+    const V3<T> p = load3(sp, i);
+    T u, v;
+    cam.project(p, u, v);
+    // Never false for a mask made by fpcl_forward; it keeps a caller-made
+    // mask from sampling outside the image.
+    const int ui = round_to_int(u), vi = round_to_int(v);
+    if (ui < 0 || ui >= W || vi < 0 || vi >= H) continue;
+    const Bilinear<T> bu0(u + h, v);
+    const Bilinear<T> bu1(std::max(u - h, T(0)), v);
+    const Bilinear<T> bv0(u, v + h);
+    const Bilinear<T> bv1(u, std::max(v - h, T(0)));
+    T gu = T(0), gv = T(0);
+    for (int c = 0; c < C; ++c) {
+      const T* ch = tf + c * HW;
+      const T du = (bu0.get(ch, H, W) - bu1.get(ch, H, W)) * div;
+      const T dv = (bv0.get(ch, H, W) - bv1.get(ch, H, W)) * div;
+      const T g = dl[c * n + i];
+      gu += du * g;
+      gv += dv * g;
+    }
+    const T z2 = p.z * p.z;
+    const T j00 = cam.fx / p.z;
+    const T j02 = -p.x * cam.fx / z2;
+    const T j11 = cam.fy / p.z;
+    const T j12 = -p.y * cam.fy / z2;
+    dx[3 * i] = j00 * gu;
+    dx[3 * i + 1] = j11 * gv;
+    dx[3 * i + 2] = j02 * gu + j12 * gv;
+  }
+}
+
+// ProjectKernel and ProjectBackwardKernel (src/camera.cu:22-29, 78-87).
+template <typename T>
+void project_forward(const T* pts, int64_t n, const T* kcam, T* uv) {
+  const Cam<T> cam(kcam);
+  for (int64_t i = 0; i < n; ++i) cam.project(load3(pts, i), uv[2 * i], uv[2 * i + 1]);
+}
+
+template <typename T>
+void project_backward(const T* dy, const T* pts, int64_t n, const T* kcam,
+                      T* dx) {
+  const Cam<T> cam(kcam);
+  for (int64_t i = 0; i < n; ++i) {
+    const V3<T> p = load3(pts, i);
+    const T z2 = p.z * p.z;
+    const T j00 = cam.fx / p.z;
+    const T j02 = -p.x * cam.fx / z2;
+    const T j11 = cam.fy / p.z;
+    const T j12 = -p.y * cam.fy / z2;
+    dx[3 * i] = j00 * dy[2 * i];
+    dx[3 * i + 1] = j11 * dy[2 * i + 1];
+    dx[3 * i + 2] = j02 * dy[2 * i] + j12 * dy[2 * i + 1];
+  }
 }
 
 // InformationMatrixKernel::operator() (src/icp_jacobian_geometric.cu:117-158):
@@ -905,5 +1007,27 @@ void orc_gaussian_blur3_f32(const float* src, int C, int H, int W,
                             double sigma, float* dst) {
   gaussian_blur3<float>(src, C, H, W, sigma, dst);
 }
+
+#define ORC_FPCL(SFX, T)                                                       \
+  void orc_fpcl_forward_##SFX(const T* tp, const T* tn, const T* tf, int C,    \
+                              int H, int W, const T* sp, int64_t n,           \
+                              const T* k, const int64_t* map, T* op, T* on,   \
+                              T* of, uint8_t* mask) {                         \
+    fpcl_forward<T>(tp, tn, tf, C, H, W, sp, n, k, map, op, on, of, mask);    \
+  }                                                                           \
+  void orc_fpcl_backward_##SFX(const T* tf, int C, int H, int W, const T* sp, \
+                               int64_t n, const uint8_t* mask, const T* dl,   \
+                               const T* k, double h, T* dx) {                 \
+    fpcl_backward<T>(tf, C, H, W, sp, n, mask, dl, k, T(h), dx);              \
+  }                                                                           \
+  void orc_project_forward_##SFX(const T* p, int64_t n, const T* k, T* uv) {  \
+    project_forward<T>(p, n, k, uv);                                          \
+  }                                                                           \
+  void orc_project_backward_##SFX(const T* dy, const T* p, int64_t n,         \
+                                  const T* k, T* dx) {                        \
+    project_backward<T>(dy, p, n, k, dx);                                     \
+  }
+ORC_FPCL(f32, float)
+ORC_FPCL(f64, double)
 
 }  // extern "C"

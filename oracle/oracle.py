@@ -188,6 +188,67 @@ def information(src_pts, src_mask, rt, tgt_pts, kcam, cmap, parallel=False):
 # SE3
 # ---------------------------------------------------------------------------
 
+def fpcl_forward(tgt_pts, tgt_nrm, tgt_feats, src_pts, kcam, cmap):
+    """FPCLMatcherOp forward: (points [N,3], normals [N,3], features [C,N],
+    mask [N] bool). Unmatched rows are zero, as in the HIP kernel (the
+    reference leaves them unwritten)."""
+    dt = np.asarray(src_pts).dtype
+    sfx = _sfx(dt)
+    tf = _c(tgt_feats, dt)
+    C, H, W = tf.shape
+    sp = _c(np.asarray(src_pts).reshape(-1, 3), dt)
+    n = sp.shape[0]
+    op = np.zeros((n, 3), dt)
+    on = np.zeros((n, 3), dt)
+    of = np.zeros((C, n), dt)
+    mask = np.zeros((n,), np.uint8)
+    k = _c(np.asarray(kcam, dtype=np.float64).reshape(3, 3), dt)
+    getattr(lib(), f"orc_fpcl_forward_{sfx}")(
+        _p(_c(tgt_pts, dt)), _p(_c(tgt_nrm, dt)), _p(tf), ctypes.c_int(C),
+        ctypes.c_int(H), ctypes.c_int(W), _p(sp), ctypes.c_int64(n), _p(k),
+        _p(_c(cmap, np.int64)), _p(op), _p(on), _p(of), _p(mask))
+    return op, on, of, mask.astype(bool)
+
+
+def fpcl_backward(tgt_feats, src_pts, mask, dl_feats, kcam, grad_precision):
+    """FPCLMatcherOp backward: d loss / d source points [N, 3]."""
+    dt = np.asarray(src_pts).dtype
+    sfx = _sfx(dt)
+    tf = _c(tgt_feats, dt)
+    C, H, W = tf.shape
+    sp = _c(np.asarray(src_pts).reshape(-1, 3), dt)
+    n = sp.shape[0]
+    dx = np.empty((n, 3), dt)
+    k = _c(np.asarray(kcam, dtype=np.float64).reshape(3, 3), dt)
+    getattr(lib(), f"orc_fpcl_backward_{sfx}")(
+        _p(tf), ctypes.c_int(C), ctypes.c_int(H), ctypes.c_int(W), _p(sp),
+        ctypes.c_int64(n), _p(_c(np.asarray(mask).reshape(-1), np.uint8)),
+        _p(_c(np.asarray(dl_feats).reshape(C, n), dt)), _p(k),
+        ctypes.c_double(grad_precision), _p(dx))
+    return dx
+
+
+def project_forward(points, kcam):
+    dt = np.asarray(points).dtype
+    p = _c(np.asarray(points).reshape(-1, 3), dt)
+    uv = np.empty((p.shape[0], 2), dt)
+    k = _c(np.asarray(kcam, dtype=np.float64).reshape(3, 3), dt)
+    getattr(lib(), f"orc_project_forward_{_sfx(dt)}")(
+        _p(p), ctypes.c_int64(p.shape[0]), _p(k), _p(uv))
+    return uv
+
+
+def project_backward(dy, points, kcam):
+    dt = np.asarray(points).dtype
+    p = _c(np.asarray(points).reshape(-1, 3), dt)
+    dx = np.empty_like(p)
+    k = _c(np.asarray(kcam, dtype=np.float64).reshape(3, 3), dt)
+    getattr(lib(), f"orc_project_backward_{_sfx(dt)}")(
+        _p(_c(np.asarray(dy).reshape(-1, 2), dt)), _p(p),
+        ctypes.c_int64(p.shape[0]), _p(k), _p(dx))
+    return dx
+
+
 def exp_rt_to_matrix(x):
     x = np.asarray(x)
     dt = x.dtype if x.dtype in (np.float32, np.float64) else np.float64

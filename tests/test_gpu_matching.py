@@ -1,0 +1,220 @@
+"""GPU parity of the differentiable matcher and projection (FPCLMatcherOp,
+ProjectOp) against the oracle, autograd checks of both, and AutogradICP on a
+synthetic pair.
+
+Bar: match masks, gathered points/normals and feature samples bit-exact;
+backward gradients bit-exact (the kernels and the oracle perform the same
+IEEE operations in the same order, contraction off); gradcheck at the
+reference's own settings (eps 1e-6, atol 1e-4).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from slamtb import synthetic
+from slamtb._cslamtb import FPCLMatcherOp as _Op, ProjectOp
+from slamtb.camera import KCamera, Project, RigidTransform
+from slamtb.frame import Frame, FrameInfo, FramePointCloud
+from slamtb.registration import ICPOdometry
+from slamtb.registration.autogradicp import AutogradICP
+from slamtb.registration.correspondence_map import CorrespondenceMap
+from slamtb.spatial.matching import FPCLMatcherOp, FramePointCloudMatcher
+
+from _helpers import prepared_pair
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+NP = {torch.float32: np.float32, torch.float64: np.float64}
+
+
+@pytest.fixture(scope="module")
+def pair():
+    k = synthetic.intrinsics(160, 120).astype(np.float32)
+    return prepared_pair(21, 160, 120, kcam=k, channels=3)
+
+
+def moved_source(P, T, dtype):
+    """Source points and normals moved by T (computed in float64 on the
+    host), valid pixels only."""
+    m = P["src_mask"].reshape(-1)
+    p = P["src_pts"].reshape(-1, 3)[m].astype(np.float64)
+    n = P["src_nrm"].reshape(-1, 3)[m].astype(np.float64)
+    p = p @ T[:3, :3].T + T[:3, 3]
+    n = n @ T[:3, :3].T
+    return p.astype(NP[dtype]), n.astype(NP[dtype])
+
+
+def t(a, dtype=None):
+    x = torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+    return x.to(dtype) if dtype is not None else x
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_fpcl_forward_and_backward_bit_exact(pair, dtype):
+    P = pair
+    T = synthetic.random_se3(np.random.default_rng(2), 2.0, 0.02) @ P["T_gt"]
+    sp, sn = moved_source(P, T, dtype)
+    npdt = NP[dtype]
+    tp, tn = P["tgt_pts"].astype(npdt), P["tgt_nrm"].astype(npdt)
+    tf = P["tgt_feats"].astype(npdt)
+    k = P["kcam"].astype(npdt)
+    # The matcher's own map: identity pose, one-element mask.
+    cmap = CorrespondenceMap(0.1, math.radians(30))(
+        t(sp), t(sn), torch.ones(1, dtype=torch.bool, device=DEV),
+        torch.eye(4, dtype=dtype, device=DEV), t(tp), t(tn), t(P["tgt_mask"]), t(k))
+    ref_map = orc.corresp_map(sp, sn, np.ones(1, np.uint8), np.eye(4), tp, tn,
+                              P["tgt_mask"], k, 0.1, math.radians(30))
+    assert np.array_equal(cmap.cpu().numpy(), ref_map)
+
+    n, C = sp.shape[0], tf.shape[0]
+    op = torch.zeros(n, 3, dtype=dtype, device=DEV)
+    on = torch.zeros(n, 3, dtype=dtype, device=DEV)
+    of = torch.zeros(C, n, dtype=dtype, device=DEV)
+    mask = torch.empty(n, dtype=torch.bool, device=DEV)
+    _Op.forward(t(tp), t(tn), t(tf), t(sp), t(k), cmap, op, on, of, mask)
+    r_op, r_on, r_of, r_mask = orc.fpcl_forward(tp, tn, tf, sp, k, ref_map)
+    print(f"{dtype}: {int(r_mask.sum())} of {n} matched")
+    assert r_mask.sum() > 0.5 * n
+    assert np.array_equal(mask.cpu().numpy(), r_mask)
+    assert np.array_equal(op.cpu().numpy(), r_op)
+    assert np.array_equal(on.cpu().numpy(), r_on)
+    assert np.array_equal(of.cpu().numpy(), r_of)
+
+    dl = np.random.default_rng(4).normal(size=(C, n)).astype(npdt)
+    dx = torch.empty(n, 3, dtype=dtype, device=DEV)
+    _Op.backward(t(tf), t(sp), mask, t(dl), t(k), 0.005, dx)
+    ref_dx = orc.fpcl_backward(tf, sp, r_mask, dl, k, 0.005)
+    assert np.array_equal(dx.cpu().numpy(), ref_dx)
+    assert np.abs(ref_dx).max() > 0
+
+
+def test_fpcl_backward_rejects_large_steps(pair):
+    tf = t(pair["tgt_feats"])
+    sp = torch.zeros(4, 3, device=DEV)
+    with pytest.raises(RuntimeError, match="grad_precision"):
+        _Op.backward(tf, sp, torch.ones(4, dtype=torch.bool, device=DEV),
+                     torch.zeros(tf.size(0), 4, device=DEV), t(pair["kcam"]), 0.5,
+                     torch.empty(4, 3, device=DEV))
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_project_bit_exact(dtype):
+    rng = np.random.default_rng(6)
+    npdt = NP[dtype]
+    p = np.stack([rng.uniform(-1, 1, 5000), rng.uniform(-1, 1, 5000),
+                  rng.uniform(0.3, 4, 5000)], 1).astype(npdt)
+    k = synthetic.intrinsics(640, 480).astype(npdt)
+    dy = rng.normal(size=(5000, 2)).astype(npdt)
+    uv = ProjectOp.forward(t(p), t(k))
+    assert np.array_equal(uv.cpu().numpy(), orc.project_forward(p, k))
+    dx = ProjectOp.backward(t(dy), t(p), t(k))
+    assert np.array_equal(dx.cpu().numpy(), orc.project_backward(dy, p, k))
+    # A [2, 3] intrinsic matrix (the reference's own test uses one).
+    uv2 = ProjectOp.forward(t(p), t(k[:2]))
+    assert torch.equal(uv, uv2)
+
+
+def test_project_gradcheck():
+    """slamtb/_test/test_camera.py:29-38 on the HIP device."""
+    torch.manual_seed(10)
+    inputs = (torch.rand(3, dtype=torch.double, device=DEV, requires_grad=True),
+              torch.tensor([[45.0, 0, 24], [0, 45, 24]], dtype=torch.double,
+                           device=DEV))
+    assert torch.autograd.gradcheck(Project.apply, inputs, eps=1e-6, atol=1e-4,
+                                    raise_exception=True)
+
+
+def test_fpcl_matcher_gradcheck(pair):
+    """spatial/_test/test_fpcl_matcher.py:13-38 on the HIP device: a 10x10
+    patch of target points nudged by up to 1 cm as the source. Target
+    features are linear in (u, v) (so the central-difference backward is
+    the exact slope); points whose projection lies within 2e-3 px of a
+    rounding boundary are left out so that the eps-perturbed forward
+    passes take the same match decisions."""
+    P = pair
+    H, W = P["tgt_mask"].shape
+    v, u = np.mgrid[0:H, 0:W].astype(np.float64)
+    feats = np.stack([0.02 * u - 0.01 * v + 1.0, 0.005 * u + 0.03 * v])
+    kc = KCamera(torch.from_numpy(P["kcam"].astype(np.float64)))
+    matcher = FramePointCloudMatcher(
+        t(P["tgt_pts"], torch.float64), t(P["tgt_nrm"], torch.float64),
+        t(P["tgt_mask"]), t(feats), kc)
+    target = matcher.target
+    gen = torch.Generator().manual_seed(10)
+    patch = P["tgt_pts"][50:60, 70:80].reshape(-1, 3).astype(np.float64)
+    patch = patch + torch.rand(patch.shape, generator=gen, dtype=torch.float64).numpy() * 0.01
+    uv = orc.project_forward(patch, P["kcam"].astype(np.float64))
+    frac = np.abs(uv - np.floor(uv) - 0.5)
+    keep = (frac > 2e-3).all(1) & (patch[:, 2] > 0)
+    patch = patch[keep]
+    normals = P["tgt_nrm"][50:60, 70:80].reshape(-1, 3)[keep]
+    src = t(patch).requires_grad_(True)
+    match = FPCLMatcherOp.Match()
+    inputs = (src, t(normals, torch.float64), target, match)
+    assert torch.autograd.gradcheck(FPCLMatcherOp.apply, inputs, eps=1e-6,
+                                    atol=1e-4, raise_exception=True)
+    assert match.mask.sum() > 0.5 * len(patch)
+
+
+def test_find_correspondences_and_rigid_transform(pair):
+    P = pair
+    T = torch.from_numpy(P["T_gt"]).to(DEV, torch.float32)
+    m = torch.from_numpy(P["src_mask"]).to(DEV)
+    sp = t(P["src_pts"])[m]
+    sn = t(P["src_nrm"])[m]
+    rig = RigidTransform(T)
+    tsp = rig @ sp
+    ref = (T[:3, :3] @ sp.T).T + T[:3, 3]
+    torch.testing.assert_close(tsp, ref, rtol=1e-6, atol=1e-6)
+    matcher = FramePointCloudMatcher(t(P["tgt_pts"]), t(P["tgt_nrm"]),
+                                     t(P["tgt_mask"]), t(P["tgt_feats"]),
+                                     KCamera(torch.from_numpy(P["kcam"])), 0.1,
+                                     math.radians(30))
+    pts, nrm, feats, mask = matcher.find_correspondences(
+        tsp, rig.transform_normals(sn))
+    assert pts.shape[0] == nrm.shape[0] == feats.shape[1] == int(mask.sum())
+    # At the true motion the matched target points sit on the moved source
+    # surface: point-to-plane residuals at the depth-noise level.
+    res = ((pts - tsp[mask]) * nrm).sum(1).abs()
+    print("matched", int(mask.sum()), "of", sp.shape[0], "median |r|",
+          float(res.median()))
+    assert mask.sum() > 0.5 * sp.shape[0]
+    assert float(res.median()) < 1e-2
+
+
+def pose_err(A, B):
+    A, B = np.asarray(A, np.float64), np.asarray(B, np.float64)
+    D = np.linalg.inv(B) @ A
+    ang = math.degrees(math.acos(min(1.0, max(-1.0, (np.trace(D[:3, :3]) - 1) / 2))))
+    return ang, float(np.linalg.norm(A[:3, 3] - B[:3, 3]))
+
+
+@pytest.mark.parametrize("use_feats", [False, True])
+def test_autogradicp_recovers_the_synthetic_motion(use_feats):
+    k = synthetic.intrinsics(160, 120).astype(np.float32)
+    P = prepared_pair(33, 160, 120, kcam=k, channels=3, noise=False)
+    kc = KCamera(torch.from_numpy(k))
+    kwargs = dict(geom_weight=1.0, feat_weight=0.0, distance_threshold=0.1,
+                  normals_angle_thresh=math.pi / 4)
+    if use_feats:
+        kwargs.update(geom_weight=0.5, feat_weight=0.5, feat_residual_thresh=2.0)
+    icp = AutogradICP(40, learning_rate=0.1, gradient_descent_iters=20, **kwargs)
+    g = {k_: t(v) for k_, v in P.items() if isinstance(v, np.ndarray) and k_ != "kcam"}
+    res = icp.estimate(kc, g["src_pts"], g["src_nrm"], g["src_mask"],
+                       target_points=g["tgt_pts"], target_mask=g["tgt_mask"],
+                       target_normals=g["tgt_nrm"],
+                       target_feats=g["tgt_feats"] if use_feats else None,
+                       source_feats=g["src_feats"] if use_feats else None)
+    err = pose_err(res.transform.numpy(), P["T_gt"])
+    start = pose_err(np.eye(4), P["T_gt"])
+    gn = ICPOdometry(20, geom_weight=1.0, feat_weight=0.0, distance_threshold=0.1,
+                     normals_angle_thresh=math.pi / 4).estimate(
+        torch.from_numpy(k), g["src_pts"], g["src_nrm"], g["src_mask"],
+        target_points=g["tgt_pts"], target_mask=g["tgt_mask"],
+        target_normals=g["tgt_nrm"])
+    print("AutogradICP", err, "start", start, "GN", pose_err(gn.transform.numpy(), P["T_gt"]))
+    assert res.transform.shape == (4, 4)
+    assert err[0] < 0.25 * start[0] and err[1] < 0.25 * start[1]
