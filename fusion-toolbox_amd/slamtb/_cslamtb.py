@@ -11,6 +11,7 @@ output convention as the pybind11 bindings:
 * ``ExpRtToMatrixOp``, ``MatrixToExpRtOp`` -- src/se3.cpp:6-15
 * ``FPCLMatcherOp`` -- src/matching.cpp:8-9
 * ``ProjectOp`` -- src/camera.cpp:11-12
+* ``RigidTransformOp`` -- src/camera.cpp:40-45
 
 Differences, all stricter or documented: tensors must live on a HIP device
 (the reference also had a CPU instantiation; this build has no CPU path),
@@ -441,6 +442,78 @@ class ProjectOp:
         return out
 
 
+def _matrix34(matrix, like):
+    m = matrix.to(device=like.device, dtype=like.dtype)
+    if m.dim() != 2 or m.size(1) != 4 or m.size(0) not in (3, 4):
+        raise RuntimeError("matrix must be [3,4] or [4,4]")
+    return m[:3].contiguous()
+
+
+class RigidTransformOp:
+    @staticmethod
+    def transform_inplace(matrix, points):
+        """src/camera.cu:134-174: points [N,3] <- R p + t in place."""
+        RigidTransformOp.transform(matrix, points, points)
+
+    @staticmethod
+    def transform_normals_inplace(matrix, normals):
+        """src/camera.cu:176-240: normals [N,3] <- inverse-transpose(R) n."""
+        RigidTransformOp.transform_normals(matrix, normals, normals)
+
+    @staticmethod
+    def transform(matrix, points, out):
+        """out [N,3] = R p + t (out may be points). Not a reference binding."""
+        what = "RigidTransformOp.transform"
+        require_gpu(points, out, what=what)
+        require_contiguous_out(points, out, what=what)
+        m = _matrix34(matrix, points)
+        n = points.numel() // 3
+        with torch.cuda.device(points.device):
+            check(lib().stb_rigid_transform_points(
+                dtype_code(points), ptr(m), ptr(points), n, ptr(out), stream()),
+                what)
+
+    @staticmethod
+    def transform_normals(matrix, normals, out):
+        """out [N,3] = inverse-transpose(R) n. Not a reference binding."""
+        what = "RigidTransformOp.transform_normals"
+        require_gpu(normals, out, what=what)
+        require_contiguous_out(normals, out, what=what)
+        m = _matrix34(matrix, normals)
+        n = normals.numel() // 3
+        with torch.cuda.device(normals.device):
+            check(lib().stb_rigid_transform_normals(
+                dtype_code(normals), ptr(m), ptr(normals), n, ptr(out), stream()),
+                what)
+
+    @staticmethod
+    def transform_backward(matrix, points, dy, dx_points, d_matrix):
+        """Backward of out = R p + t: dx_points [N,3] = R^T dy (None skips
+        it) and d_matrix [3,4] = sum_i dy_i [p_i; 1]^T. Not a reference
+        binding (the reference differentiates a torch matmul)."""
+        what = "RigidTransform backward"
+        require_gpu(points, dy, dx_points, d_matrix, what=what)
+        require_contiguous_out(dx_points, d_matrix, what=what)
+        p = _in(points).view(-1, 3)
+        g = _in(dy, p.dtype).view(-1, 3)
+        m = _matrix34(matrix, p)
+        ws_bytes = lib().stb_rigid_backward_workspace_bytes()
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=p.device)
+        with torch.cuda.device(p.device):
+            check(lib().stb_rigid_transform_backward(
+                dtype_code(p), ptr(m), ptr(p), p.size(0), ptr(g),
+                ptr(dx_points), ptr(d_matrix), ptr(ws), ws_bytes, stream()),
+                what)
+
+    @staticmethod
+    def rodrigues(rot_matrix, rodrigues):
+        """src/camera.cpp:15-37 is CPU-only and fills only element (0, 0) of
+        its rotation before taking the axis, so its output does not depend
+        on the input's rotation; it is not provided."""
+        raise NotImplementedError("RigidTransformOp.rodrigues is not provided")
+
+
 __all__ = ["CorrespondenceMap", "ICPJacobian", "Processing",
            "EstimateNormalsMethod", "DownsampleXYZMethod", "ExpRtToMatrixOp",
-           "MatrixToExpRtOp", "FPCLMatcherOp", "ProjectOp", "_lib"]
+           "MatrixToExpRtOp", "FPCLMatcherOp", "ProjectOp", "RigidTransformOp",
+           "_lib"]

@@ -126,6 +126,11 @@ _SIGNATURES = {
         _i32, _vp, _i32, _i32, _i32, _vp, _i64, _vp, _vp, _vp, _f64, _vp, _vp]),
     "stb_project_forward": (ctypes.c_int, [_i32, _vp, _i64, _vp, _vp, _vp]),
     "stb_project_backward": (ctypes.c_int, [_i32, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "stb_rigid_transform_points": (ctypes.c_int, [_i32, _vp, _vp, _i64, _vp, _vp]),
+    "stb_rigid_transform_normals": (ctypes.c_int, [_i32, _vp, _vp, _i64, _vp, _vp]),
+    "stb_rigid_backward_workspace_bytes": (_sz, []),
+    "stb_rigid_transform_backward": (ctypes.c_int, [
+        _i32, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _sz, _vp]),
     "stb_icp_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "stb_icp_begin": (ctypes.c_int, [_vp, _sz, _vp, _i32, _i64, _i32, _i32, _vp]),
     "stb_icp_iterate": (ctypes.c_int, [

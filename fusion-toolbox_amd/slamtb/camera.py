@@ -4,7 +4,7 @@ import math
 
 import torch
 
-from ._cslamtb import ProjectOp as _ProjectOp
+from ._cslamtb import ProjectOp as _ProjectOp, RigidTransformOp as _RigidTransformOp
 from ._utils import ensure_torch
 
 
@@ -99,31 +99,85 @@ class Project(torch.autograd.Function):
         return _ProjectOp.backward(dy_grad, points, intrinsics), None
 
 
+class _RigidPoints(torch.autograd.Function):
+    """p' = R p + t on the HIP device, differentiable in the matrix and the
+    points: one transform launch forward, one transpose-and-reduce launch
+    pair backward (FP64 sums in a fixed order)."""
+
+    @staticmethod
+    def forward(ctx, matrix, points):
+        out = torch.empty_like(points)
+        _RigidTransformOp.transform(matrix, points, out)
+        ctx.save_for_backward(matrix, points)
+        return out
+
+    @staticmethod
+    def backward(ctx, dy):
+        matrix, points = ctx.saved_tensors
+        dx = torch.empty_like(points) if ctx.needs_input_grad[1] else None
+        d34 = torch.empty(3, 4, dtype=points.dtype, device=points.device)
+        _RigidTransformOp.transform_backward(matrix, points, dy, dx, d34)
+        dm = None
+        if ctx.needs_input_grad[0]:
+            dm = d34 if matrix.size(0) == 3 else torch.cat(
+                [d34, torch.zeros(1, 4, dtype=d34.dtype, device=d34.device)])
+            dm = dm.to(matrix.dtype)
+        return dm, dx
+
+
+def _on_gpu(*tensors):
+    return all(t.is_cuda and t.dtype in (torch.float32, torch.float64)
+               for t in tensors)
+
+
 class RigidTransform:
     """A [4, 4] or [3, 4] rigid matrix applied to [N, 3] points with ``@``
-    (camera.py:311-403)."""
+    (camera.py:311-403).
+
+    On HIP tensors ``@`` is one transform kernel (differentiable in matrix
+    and points) and ``transform_normals`` of tensors that need no gradient
+    is one kernel with the cofactor inverse-transpose of the reference's
+    native RigidTransform (include/slamtb/camera.hpp:99-116); a normal
+    transform that needs a gradient uses torch's inverse, as the reference
+    does. CPU tensors take the reference's torch expressions."""
 
     def __init__(self, matrix):
         self.matrix = matrix
         self._normal_matrix = None
 
     def __matmul__(self, points):
+        if _on_gpu(points):
+            m = self.matrix.to(points.device, points.dtype)
+            return _RigidPoints.apply(m, points.reshape(-1, 3).contiguous()).squeeze()
         points = self.matrix[:3, :3].matmul(points.view(-1, 3, 1))
         points += self.matrix[:3, 3].view(3, 1)
         return points.squeeze()
 
     def transform_normals(self, normals):
         """Normals through the inverse transpose of the rotation block."""
+        if (_on_gpu(normals) and not normals.requires_grad
+                and not (torch.is_grad_enabled() and self.matrix.requires_grad)):
+            n = normals.reshape(-1, 3).contiguous()
+            out = torch.empty_like(n)
+            _RigidTransformOp.transform_normals(self.matrix.detach(), n, out)
+            return out
         return (self.normal_matrix @ normals.view(-1, 3, 1)).view(-1, 3)
 
     def inplace(self, points):
         points = points.view(-1, 3)
-        points.copy_(self @ points)
+        if _on_gpu(points):
+            _RigidTransformOp.transform_inplace(self.matrix, points)
+        else:
+            points.copy_(self.matrix[:3, :3].matmul(points.view(-1, 3, 1)).view(-1, 3)
+                         + self.matrix[:3, 3])
         return points
 
     def inplace_normals(self, normals):
         normals = normals.view(-1, 3)
-        normals.copy_(self.transform_normals(normals))
+        if _on_gpu(normals):
+            _RigidTransformOp.transform_normals_inplace(self.matrix, normals)
+        else:
+            normals.copy_(self.transform_normals(normals))
         return normals
 
     @property

@@ -73,6 +73,64 @@ class _HuberLoss(torch.nn.Module):
         return loss
 
 
+def _compacted_loss(tsrc, tpoints, tnormals, tfeatures, smask, source_feats,
+                    icp, huber_loss):
+    """The reference's cost on compacted matches (autogradicp.py:183-201):
+    returns (geometric loss, feature loss); 0 for a disabled term."""
+    geom_loss = 0
+    feat_loss = 0
+    if icp.geom_weight > 0:
+        diff = tpoints - tsrc[smask]
+        cost = (tnormals * diff).sum(1)
+        geom_loss = torch.pow(cost, 2).mean()
+    if icp.feat_weight > 0 and source_feats is not None:
+        feat_diff = torch.norm(tfeatures - source_feats[:, smask], 2, dim=0)
+        if icp.feat_residual_thresh > 0:
+            thr2 = icp.feat_residual_thresh * icp.feat_residual_thresh
+            feat_diff = feat_diff[feat_diff.pow(2) < thr2]
+        if icp.huber_loss_alpha > 0:
+            feat_loss = huber_loss(feat_diff)
+        else:
+            feat_loss = feat_diff.mean()
+    return geom_loss, feat_loss
+
+
+def _masked_mean(values, keep):
+    """Mean of values[keep] as a masked sum over the count; NaN when nothing
+    is kept, as the mean of an empty selection."""
+    return torch.where(keep, values, 0).sum() / keep.sum()
+
+
+def _row_loss(tsrc, match, source_feats, icp):
+    """The same cost on uncompacted match rows, as masked sums: no host
+    synchronisation. Equal to :func:`_compacted_loss` up to summation
+    order; returns (geometric loss, feature loss)."""
+    mask = match.mask
+    geom_loss = 0
+    feat_loss = 0
+    if icp.geom_weight > 0:
+        cost = ((match.points - tsrc) * match.normals).sum(1)
+        geom_loss = _masked_mean(cost.pow(2), mask)
+    if icp.feat_weight > 0 and source_feats is not None:
+        d = torch.norm(match.features - source_feats, 2, dim=0)
+        keep = mask
+        if icp.feat_residual_thresh > 0:
+            thr2 = icp.feat_residual_thresh * icp.feat_residual_thresh
+            keep = keep & (d.detach().pow(2) < thr2)
+        alpha = icp.huber_loss_alpha
+        if alpha > 0:
+            # Each Huber part counts only when it selected something
+            # (autogradicp.py:84-88); with nothing selected its sum is 0.
+            lo = keep & (d.detach() < alpha)
+            hi = keep & (d.detach() >= alpha)
+            feat_loss = (torch.where(lo, d, 0).sum() / lo.sum().clamp(min=1)
+                         + torch.where(hi, alpha * torch.abs(d) - alpha, 0).sum()
+                         / hi.sum().clamp(min=1))
+        else:
+            feat_loss = _masked_mean(d, keep)
+    return geom_loss, feat_loss
+
+
 class AutogradICP:
     """ICP whose 6-D gradient comes from autograd (autogradicp.py:95-374).
 
@@ -82,7 +140,12 @@ class AutogradICP:
     ``distance_threshold`` / ``normals_angle_thresh`` gate matches,
     ``feat_residual_thresh`` drops feature residuals at or above it,
     ``huber_loss_alpha`` (0 disables Huber) and ``gradient_descent_iters``
-    SGD refinement steps (None skips them)."""
+    SGD refinement steps (None skips them).
+
+    With a frame matcher each cost evaluation runs on the device without a
+    host round trip (masked sums over the uncompacted match rows) until
+    scipy or the SGD loop reads the cost; another matcher is used through
+    ``find_correspondences`` and the reference's compacted cost."""
 
     def __init__(self, num_iters, learning_rate=0.01, geom_weight=0.5,
                  feat_weight=0.5, distance_threshold=0.1,
@@ -111,41 +174,30 @@ class AutogradICP:
         exp_rt.requires_grad_(True)
 
         loss = math.inf
-        has_geom = self.geom_weight > 0
-        has_feat = self.feat_weight > 0 and source_feats is not None
+        if self.feat_weight <= 0:
+            source_feats = None
         huber_loss = _HuberLoss(self.huber_loss_alpha)
+        rows = hasattr(target_matcher, "match_rows")
 
         def _closure():
             nonlocal loss
             if exp_rt.grad is not None:
                 exp_rt.grad.zero_()
-            geom_loss = 0
-            feat_loss = 0
 
             rigid = RigidTransform(ExpRtToMatrix.apply(exp_rt))
-            tsrc = rigid @ source_points
-            tpoints, tnormals, tfeatures, smask = \
-                target_matcher.find_correspondences(
-                    tsrc, rigid.transform_normals(source_normals))
-
-            if has_geom:
-                diff = tpoints - tsrc[smask]
-                cost = (tnormals * diff).sum(1)
-                geom_loss = torch.pow(cost, 2).mean()
-
-            if has_feat:
-                feat_diff = torch.norm(tfeatures - source_feats[:, smask], 2,
-                                       dim=0)
-                if self.feat_residual_thresh > 0:
-                    thr2 = self.feat_residual_thresh * self.feat_residual_thresh
-                    feat_diff = feat_diff[feat_diff.pow(2) < thr2]
-                if self.huber_loss_alpha > 0:
-                    feat_loss = huber_loss(feat_diff)
-                else:
-                    feat_loss = feat_diff.mean()
+            tsrc = (rigid @ source_points).view(-1, 3)
+            with torch.no_grad():
+                tnormals = rigid.transform_normals(source_normals)
+            if rows:
+                match = target_matcher.match_rows(tsrc, tnormals)
+                geom_loss, feat_loss = _row_loss(tsrc, match, source_feats, self)
+            else:
+                geom_loss, feat_loss = _compacted_loss(
+                    tsrc, *target_matcher.find_correspondences(tsrc, tnormals),
+                    source_feats, self, huber_loss)
 
             loss = geom_loss * self.geom_weight + feat_loss * self.feat_weight
-            if not torch.is_tensor(loss) or torch.isnan(loss):
+            if not torch.is_tensor(loss):
                 return loss
             return loss * self.learning_rate
 

@@ -110,10 +110,20 @@ class FramePointCloudMatcher:
     def find_correspondences(self, source_points, source_normals):
         """Returns (points [M, 3], normals [M, 3], features [C, M], mask [N])
         of the M matched source points."""
-        match = FPCLMatcherOp.Match()
-        FPCLMatcherOp.apply(source_points, source_normals, self.target, match)
+        match = self.match_rows(source_points, source_normals)
         return (match.points[match.mask, :], match.normals[match.mask, :],
                 match.features[:, match.mask], match.mask)
+
+    def match_rows(self, source_points, source_normals):
+        """The uncompacted match: a ``FPCLMatcherOp.Match`` whose points and
+        normals are [N, 3] and features [C, N] (zero rows where ``mask`` is
+        false), the features differentiable in ``source_points``. Selecting
+        rows by the mask needs the match count on the host; callers that
+        can work with masked sums avoid that synchronisation."""
+        match = FPCLMatcherOp.Match()
+        match.features = FPCLMatcherOp.apply(source_points, source_normals,
+                                             self.target, match)
+        return match
 
 
 class PointCloudMatcher:

@@ -207,6 +207,34 @@ int stb_project_backward(int32_t dtype, const void* dy_uv, const void* points,
                          int64_t n, const void* intrinsics, void* dx_points,
                          void* stream);
 
+/* RigidTransformOp.transform_inplace (src/camera.cpp:43, src/camera.cu:
+ * 134-174): out[n,3] = R p + t for points[n,3]; matrix is the device-resident
+ * row-major 3x4 block (rows 0..2 of a [3,4] or [4,4] tensor, row stride 4).
+ * out may equal points (in place). Also the forward pass of RigidTransform's
+ * `@` (slamtb/camera.py:333-337). */
+int stb_rigid_transform_points(int32_t dtype, const void* matrix,
+                               const void* points, int64_t n, void* out,
+                               void* stream);
+
+/* RigidTransformOp.transform_normals_inplace (src/camera.cpp:44,
+ * src/camera.cu:176-240): out[n,3] = N n with N the cofactor
+ * inverse-transpose of R (include/slamtb/camera.hpp:99-116). */
+int stb_rigid_transform_normals(int32_t dtype, const void* matrix,
+                                const void* normals, int64_t n, void* out,
+                                void* stream);
+
+/* Backward of out = R p + t (no native counterpart: the reference
+ * differentiates its torch matmul, slamtb/camera.py:333-337):
+ * dx_points[n,3] = R^T dy (skipped when NULL) and d_matrix[3,4] =
+ * sum_i dy_i [p_i; 1]^T, summed in FP64 in a fixed order. workspace is device
+ * memory of at least stb_rigid_backward_workspace_bytes() bytes. */
+size_t stb_rigid_backward_workspace_bytes(void);
+int stb_rigid_transform_backward(int32_t dtype, const void* matrix,
+                                 const void* points, int64_t n, const void* dy,
+                                 void* dx_points, void* d_matrix,
+                                 void* workspace, size_t workspace_bytes,
+                                 void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Fused Gauss-Newton path used by slamtb.registration.ICPOdometry.          */
 /* It replaces the per-iteration sequence of icp.py:210-256 (two maps, two   */

@@ -31,6 +31,9 @@
 //     inputs (constant depth, planes, spheres, hand bitmaps);
 //   gaussian blur 3x3: pinned only against the normalised Gaussian formula;
 //     the border rule of the reference's kornia call is parity unpinned;
+//   rigid_points / rigid_backward: pinned against matrix products, the
+//     inverse-transpose of random matrices and finite differences
+//     (test_oracle_matching.py);
 //   fpcl_forward / fpcl_backward / project_*: pinned by closed forms
 //     (features linear in u and v, whose bilinear sample and central
 //     difference are exact), a brute-force matcher over the map, and finite
@@ -476,6 +479,42 @@ void project_backward(const T* dy, const T* pts, int64_t n, const T* kcam,
     dx[3 * i + 1] = j11 * dy[2 * i + 1];
     dx[3 * i + 2] = j02 * dy[2 * i] + j12 * dy[2 * i + 1];
   }
+}
+
+// RigidTransformOp (src/camera.cu:134-240): points through [R | t] and
+// normals through the cofactor inverse-transpose, and the backward of the
+// point transform: dx = R^T dy, dM[r][c] = sum_i dy_i[r] [p_i; 1][c] summed
+// sequentially in double.
+template <typename T>
+void rigid_points(const T* m, const T* in, int64_t n, T* out, int normals) {
+  const Rigid<T> rig(m);
+  for (int64_t i = 0; i < n; ++i) {
+    const V3<T> p = load3(in, i);
+    const V3<T> q = normals ? rig.transform_normal(p) : rig.transform(p);
+    out[3 * i] = q.x;
+    out[3 * i + 1] = q.y;
+    out[3 * i + 2] = q.z;
+  }
+}
+
+template <typename T>
+void rigid_backward(const T* m, const T* p, const T* dy, int64_t n, T* dx,
+                    T* dM) {
+  double acc[12] = {0};
+  for (int64_t i = 0; i < n; ++i) {
+    const V3<T> g = load3(dy, i);
+    const V3<T> q = load3(p, i);
+    dx[3 * i] = m[0] * g.x + m[4] * g.y + m[8] * g.z;
+    dx[3 * i + 1] = m[1] * g.x + m[5] * g.y + m[9] * g.z;
+    dx[3 * i + 2] = m[2] * g.x + m[6] * g.y + m[10] * g.z;
+    const double gr[3] = {double(g.x), double(g.y), double(g.z)};
+    const double pc[3] = {double(q.x), double(q.y), double(q.z)};
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) acc[r * 4 + c] += gr[r] * pc[c];
+      acc[r * 4 + 3] += gr[r];
+    }
+  }
+  for (int k = 0; k < 12; ++k) dM[k] = T(acc[k]);
 }
 
 // InformationMatrixKernel::operator() (src/icp_jacobian_geometric.cu:117-158):
@@ -1029,5 +1068,17 @@ void orc_gaussian_blur3_f32(const float* src, int C, int H, int W,
   }
 ORC_FPCL(f32, float)
 ORC_FPCL(f64, double)
+
+#define ORC_RIGID(SFX, T)                                                      \
+  void orc_rigid_points_##SFX(const T* m, const T* in, int64_t n, T* out,     \
+                              int normals) {                                  \
+    rigid_points<T>(m, in, n, out, normals);                                  \
+  }                                                                           \
+  void orc_rigid_backward_##SFX(const T* m, const T* p, const T* dy,          \
+                                int64_t n, T* dx, T* dM) {                    \
+    rigid_backward<T>(m, p, dy, n, dx, dM);                                   \
+  }
+ORC_RIGID(f32, float)
+ORC_RIGID(f64, double)
 
 }  // extern "C"

@@ -249,6 +249,30 @@ def project_backward(dy, points, kcam):
     return dx
 
 
+def rigid_points(matrix, points, normals=False):
+    """RigidTransformOp: [N,3] points through [R | t], or normals through
+    the cofactor inverse-transpose of R."""
+    dt = np.asarray(points).dtype
+    p = _c(np.asarray(points).reshape(-1, 3), dt)
+    out = np.empty_like(p)
+    getattr(lib(), f"orc_rigid_points_{_sfx(dt)}")(
+        _p(_rt34(matrix, dt)), _p(p), ctypes.c_int64(p.shape[0]), _p(out),
+        ctypes.c_int(int(normals)))
+    return out
+
+
+def rigid_backward(matrix, points, dy):
+    """Backward of p' = R p + t: (dx [N,3], dM [3,4])."""
+    dt = np.asarray(points).dtype
+    p = _c(np.asarray(points).reshape(-1, 3), dt)
+    dx = np.empty_like(p)
+    dM = np.empty((3, 4), dt)
+    getattr(lib(), f"orc_rigid_backward_{_sfx(dt)}")(
+        _p(_rt34(matrix, dt)), _p(p), _p(_c(np.asarray(dy).reshape(-1, 3), dt)),
+        ctypes.c_int64(p.shape[0]), _p(dx), _p(dM))
+    return dx, dM
+
+
 def exp_rt_to_matrix(x):
     x = np.asarray(x)
     dt = x.dtype if x.dtype in (np.float32, np.float64) else np.float64

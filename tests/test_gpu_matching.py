@@ -15,7 +15,7 @@ import torch
 
 from oracle import oracle as orc
 from slamtb import synthetic
-from slamtb._cslamtb import FPCLMatcherOp as _Op, ProjectOp
+from slamtb._cslamtb import FPCLMatcherOp as _Op, ProjectOp, RigidTransformOp
 from slamtb.camera import KCamera, Project, RigidTransform
 from slamtb.frame import Frame, FrameInfo, FramePointCloud
 from slamtb.registration import ICPOdometry
@@ -218,3 +218,48 @@ def test_autogradicp_recovers_the_synthetic_motion(use_feats):
     print("AutogradICP", err, "start", start, "GN", pose_err(gn.transform.numpy(), P["T_gt"]))
     assert res.transform.shape == (4, 4)
     assert err[0] < 0.25 * start[0] and err[1] < 0.25 * start[1]
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_rigid_transform_kernels_bit_exact(dtype):
+    rng = np.random.default_rng(12)
+    npdt = NP[dtype]
+    T = synthetic.random_se3(rng, 20.0, 0.5)
+    T[:3, :3] *= 1.3  # not orthonormal: the normal matrix is a true inverse
+    M = T.astype(npdt)
+    p = rng.normal(size=(70001, 3)).astype(npdt)
+    dy = rng.normal(size=(70001, 3)).astype(npdt)
+    out = torch.empty(70001, 3, dtype=dtype, device=DEV)
+    RigidTransformOp.transform(t(M), t(p), out)
+    assert np.array_equal(out.cpu().numpy(), orc.rigid_points(M, p))
+    RigidTransformOp.transform_normals(t(M), t(p), out)
+    assert np.array_equal(out.cpu().numpy(), orc.rigid_points(M, p, normals=True))
+    inplace = t(p)
+    RigidTransformOp.transform_inplace(t(M[:3]), inplace)
+    assert np.array_equal(inplace.cpu().numpy(), orc.rigid_points(M, p))
+    dx = torch.empty_like(out)
+    dM = torch.empty(3, 4, dtype=dtype, device=DEV)
+    RigidTransformOp.transform_backward(t(M), t(p), t(dy), dx, dM)
+    rdx, rdM = orc.rigid_backward(M, p, dy)
+    assert np.array_equal(dx.cpu().numpy(), rdx)
+    # FP64 sums of 70001 terms in another order: equal to the FP64
+    # accumulation error (n eps sum|terms|), then rounded to dtype.
+    bound = 70001 * np.finfo(np.float64).eps * np.abs(p).max() * np.abs(dy).max() * 4
+    np.testing.assert_allclose(dM.cpu().numpy(), rdM,
+                               rtol=2 * np.finfo(npdt).eps, atol=bound)
+
+
+def test_rigid_transform_autograd():
+    torch.manual_seed(3)
+    M = torch.randn(4, 4, dtype=torch.float64, device=DEV, requires_grad=True)
+    p = torch.randn(40, 3, dtype=torch.float64, device=DEV, requires_grad=True)
+    assert torch.autograd.gradcheck(lambda m, q: RigidTransform(m) @ q, (M, p),
+                                    eps=1e-6, atol=1e-8)
+    M3 = torch.randn(3, 4, dtype=torch.float64, device=DEV, requires_grad=True)
+    assert torch.autograd.gradcheck(lambda m, q: RigidTransform(m) @ q, (M3, p),
+                                    eps=1e-6, atol=1e-8)
+    # The reference's expression (camera.py:333-337) on the CPU.
+    Mc, pc = M.detach().cpu(), p.detach().cpu()
+    ref = (Mc[:3, :3].matmul(pc.view(-1, 3, 1)) + Mc[:3, 3].view(3, 1)).squeeze()
+    torch.testing.assert_close((RigidTransform(M.detach()) @ p.detach()).cpu(), ref,
+                               rtol=1e-12, atol=1e-12)

@@ -1,4 +1,5 @@
-"""Pins the oracle's FPCLMatcherOp and ProjectOp restatements (CPU only).
+"""Pins the oracle's FPCLMatcherOp, ProjectOp and RigidTransformOp
+restatements (CPU only).
 
 * Projection: the closed form fx*x/z + cx on hand points, and the backward
   pass against central differences of the forward pass.
@@ -11,6 +12,8 @@
   exact slope, so d loss/d point is J_project^T (sum_c a_c g_c, sum_c b_c g_c);
   at u < h the lower sample clamps to column 0, which changes the slope to
   f(u + h) - f(0) over 2h.
+* Rigid transforms: against float64 matrix products and the inverse-transpose
+  from numpy's inverse; the backward pass against finite differences.
 """
 import math
 
@@ -190,3 +193,41 @@ def test_fpcl_backward_float32_follows_float64():
     d32 = orc.fpcl_backward(tf.astype(np.float32), src.astype(np.float32), mask,
                             dl.astype(np.float32), K, 0.05)
     np.testing.assert_allclose(d32, d64, rtol=2e-3, atol=2e-2 * np.abs(d64).max())
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rigid_points_and_normals(dtype):
+    rng = np.random.default_rng(9)
+    M = rng.normal(size=(3, 4))
+    p = rng.normal(size=(100, 3)).astype(dtype)
+    got = orc.rigid_points(M, p)
+    exp = p.astype(np.float64) @ M[:, :3].T + M[:, 3]
+    np.testing.assert_allclose(got, exp, rtol=1e-5 if dtype == np.float32 else 1e-13,
+                               atol=1e-5 if dtype == np.float32 else 1e-13)
+    nrm = orc.rigid_points(M, p, normals=True)
+    N = np.linalg.inv(M[:, :3]).T
+    np.testing.assert_allclose(nrm, p.astype(np.float64) @ N.T,
+                               rtol=1e-4 if dtype == np.float32 else 1e-11,
+                               atol=1e-4 if dtype == np.float32 else 1e-11)
+
+
+def test_rigid_backward_matches_finite_differences():
+    rng = np.random.default_rng(10)
+    M = rng.normal(size=(3, 4))
+    p = rng.normal(size=(50, 3))
+    dy = rng.normal(size=(50, 3))
+    dx, dM = orc.rigid_backward(M, p, dy)
+    loss = lambda MM, pp: float((orc.rigid_points(MM, pp) * dy).sum())
+    eps = 1e-6
+    for r in range(3):
+        for c in range(4):
+            E = np.zeros((3, 4))
+            E[r, c] = eps
+            num = (loss(M + E, p) - loss(M - E, p)) / (2 * eps)
+            assert abs(num - dM[r, c]) < 1e-6 * max(1, abs(num))
+    for i in (0, 17, 49):
+        for j in range(3):
+            P = np.zeros_like(p)
+            P[i, j] = eps
+            num = (loss(M, p + P) - loss(M, p - P)) / (2 * eps)
+            assert abs(num - dx[i, j]) < 1e-6 * max(1, abs(num))
