@@ -33,6 +33,9 @@ synchronisations. Also reported:
     adjoint backward;
   * the differentiable frame matcher and projection kernels at 640x480 and
     AutogradICP driven by them, with the oracle's matcher beside them;
+  * the reference's own RGB-D test frames: the Registration notebook's pair
+    against the camera poses recorded with the frames, and frame-to-frame
+    tracking over 15 frames;
   * the CPU baseline: the oracle's OpenMP port of the reference CPU path on
     this host, at all threads and at one thread.
 """
@@ -69,6 +72,8 @@ def parse():
                     help="pairs of the batched workload (0 disables it)")
     ap.add_argument("--no-bf16", action="store_true",
                     help="skip the 1280x720 BF16-storage workload")
+    ap.add_argument("--no-ftb", action="store_true",
+                    help="skip the real-frame (FTB sample1) section")
     ap.add_argument("--no-matcher", action="store_true",
                     help="skip the differentiable matcher / AutogradICP section")
     ap.add_argument("--no-diff", action="store_true",
@@ -605,6 +610,80 @@ def matcher_section(pair, top, dev, with_cpu):
     return out
 
 
+def ftb_section(dev):
+    """The reference's own RGB-D frames (FTB copies under tests/golden/ftb):
+    the Registration notebook's pair, sample1 0 -> 14 with RGB features
+    (10/15/20 iterations, weights 1 : 1), and depth-only frame-to-frame
+    tracking over frames 0..14; accuracy against the recorded camera poses
+    and the time per registration (preprocessing on the GPU included,
+    PNG decoding excluded)."""
+    import torch
+    from slamtb import metrics
+    from slamtb.camera import RTCamera
+    from slamtb.data import load_ftb
+    from slamtb.frame import Frame, FramePointCloud
+    from slamtb.processing import bilateral_depth_filter, to_color_feature
+    from slamtb.registration import ICPOdometry, MultiscaleRegistration
+
+    ds = load_ftb(ROOT / "tests" / "golden" / "ftb" / "sample1")
+    frames = [ds[i] for i in range(len(ds))]
+    depth = [torch.from_numpy(f.depth_image).to(dev) for f in frames]
+
+    def fpcl(i):
+        d = bilateral_depth_filter(depth[i])
+        return FramePointCloud.from_frame(Frame(frames[i].info, d), device=dev)
+
+    kw = dict(geom_weight=1.0, feat_weight=1.0, distance_threshold=0.5,
+              normals_angle_thresh=math.pi / 4, feat_residual_thresh=2.75)
+    ms = MultiscaleRegistration([(1.0, ICPOdometry(10, **kw)),
+                                 (0.5, ICPOdometry(15, **kw)),
+                                 (0.5, ICPOdometry(20, **kw))])
+    sf = to_color_feature(frames[14].rgb_image).to(dev)
+    tf = to_color_feature(frames[0].rgb_image).to(dev)
+    pair = lambda: ms.estimate_frame(fpcl(14), fpcl(0), source_feats=sf,
+                                     target_feats=tf)
+    res = pair()
+    walls = []
+    for _ in range(10):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = pair()
+        walls.append(time.perf_counter() - t0)
+    gt = {0: frames[0].info.rt_cam, 1: frames[14].info.rt_cam}
+    pred = {0: RTCamera(), 1: RTCamera(res.transform)}
+    out = {"notebook_pair_0_14": {
+        "rte_m": metrics.relative_translational_error(gt, pred).item(),
+        "rre_rad": metrics.relative_rotational_error(gt, pred).item(),
+        "notebook_rte_m": 0.011427460885717639,
+        "notebook_rre_rad": 0.015468830242753029,
+        "ms_per_registration": statistics.median(walls) * 1e3}}
+
+    kw = dict(geom_weight=1.0, feat_weight=0.0, distance_threshold=0.1,
+              normals_angle_thresh=math.pi / 4)
+    ms = MultiscaleRegistration([(1.0, ICPOdometry(10, **kw)),
+                                 (0.5, ICPOdometry(10, **kw)),
+                                 (0.5, ICPOdometry(10, **kw))])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prev = fpcl(0)
+    pose = torch.eye(4, dtype=torch.float64)
+    gt, pred = {0: frames[0].info.rt_cam}, {0: RTCamera(pose)}
+    for i in range(1, len(frames)):
+        cur = fpcl(i)
+        pose = pose @ ms.estimate_frame(cur, prev).transform
+        gt[i], pred[i] = frames[i].info.rt_cam, RTCamera(pose)
+        prev = cur
+    el = time.perf_counter() - t0
+    gt = metrics.set_start_at_identity(gt)
+    rmse = lambda e: e.pow(2).mean().sqrt().item()
+    out["tracking_0_14_depth"] = {
+        "ate_rmse_m": rmse(metrics.absolute_translational_error(gt, pred)),
+        "rte_rmse_m": rmse(metrics.relative_translational_error(gt, pred)),
+        "rre_rmse_rad": rmse(metrics.relative_rotational_error(gt, pred)),
+        "ms_per_frame": el * 1e3 / (len(frames) - 1)}
+    return out
+
+
 def main():
     args = parse()
     if args.copy_probe > 0:
@@ -619,7 +698,7 @@ def main():
     from slamtb.registration._fused import runner_for
     if args.profile:
         args.no_oracle = args.no_cpu_baseline = args.no_bf16 = True
-        args.no_diff = args.no_matcher = True
+        args.no_diff = args.no_matcher = args.no_ftb = True
         args.pairs = 0
 
     dev = torch.device("cuda:0")
@@ -735,6 +814,9 @@ def main():
     if not args.no_bf16:
         log("1280x720 BF16 / FP32 storage workload")
         out["bf16_1280x720"] = bf16_section(args.seed, dev, not args.no_oracle)
+    if not args.no_ftb:
+        log("reference RGB-D frames (FTB sample1)")
+        out["ftb_sample1"] = ftb_section(dev)
     if not args.no_matcher:
         log("differentiable matcher and AutogradICP at 640x480")
         out["matcher_640x480"] = matcher_section(pair, top, dev, not args.no_oracle)

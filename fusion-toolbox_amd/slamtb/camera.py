@@ -30,6 +30,21 @@ class KCamera:
         return cls(k_trans @ k_scale, undist_coeff, image_size)
 
     @classmethod
+    def from_json(cls, json):
+        """FTB JSON form (camera.py:45-52)."""
+        return cls(torch.tensor(json['matrix'], dtype=torch.float).view(-1, 3),
+                   undist_coeff=json.get('undist_coeff', None),
+                   image_size=json.get('image_size', None))
+
+    def to_json(self):
+        json = {'matrix': self.matrix.tolist()}
+        if self.undist_coeff is not None:
+            json['undist_coeff'] = self.undist_coeff
+        if self.image_size is not None:
+            json['image_size'] = list(self.image_size)
+        return json
+
+    @classmethod
     def from_estimation_by_fov(cls, hfov, vfov, img_width, img_height):
         flen_x = img_width * .5 / math.tan(hfov / 2)
         flen_y = img_height * .5 / math.tan(vfov / 2)
@@ -81,6 +96,93 @@ class KCamera:
 
     def __str__(self):
         return f"KCamera({self.matrix.tolist()})"
+
+
+class RTCamera:
+    """Extrinsic camera: ``matrix`` is the [4, 4] float64 camera-to-world
+    transform (camera.py:420-607)."""
+
+    def __init__(self, matrix=None):
+        if matrix is not None:
+            self.matrix = ensure_torch(matrix, dtype=torch.double)
+        else:
+            self.matrix = torch.eye(4, dtype=torch.double)
+
+    @classmethod
+    def create_from_pos_rot(cls, position, rotation_matrix):
+        m = torch.eye(4, dtype=torch.double)
+        m[:3, :3] = torch.as_tensor(rotation_matrix, dtype=torch.double)
+        m[:3, 3] = torch.as_tensor(position, dtype=torch.double)
+        return cls(m)
+
+    @classmethod
+    def create_from_pos_quat(cls, x, y, z, qw, qx, qy, qz):
+        m = torch.eye(4, dtype=torch.double)
+        m[:3, :3] = quaternion_to_matrix(qw, qx, qy, qz)
+        m[:3, 3] = torch.tensor([x, y, z], dtype=torch.double)
+        return cls(m)
+
+    @classmethod
+    def from_json(cls, json):
+        return cls(torch.tensor(json['matrix'], dtype=torch.double).view(-1, 4))
+
+    def to_json(self):
+        return {'matrix': self.matrix.tolist()}
+
+    @property
+    def cam_to_world(self):
+        return self.matrix
+
+    @property
+    def world_to_cam(self):
+        return torch.inverse(self.matrix)
+
+    def right_transform(self, matrix):
+        """matrix @ self.matrix (camera.py:525-535)."""
+        return RTCamera(torch.as_tensor(matrix).cpu().double() @ self.matrix)
+
+    def transform(self, matrix):
+        """self.matrix @ matrix (camera.py:537-547)."""
+        return RTCamera(self.matrix @ torch.as_tensor(matrix).cpu().double())
+
+    def __matmul__(self, other):
+        return RTCamera(self.matrix @ other.matrix.double())
+
+    def translate(self, tx, ty, tz):
+        t = torch.eye(4, dtype=torch.double)
+        t[:3, 3] = torch.tensor([tx, ty, tz], dtype=torch.double)
+        return RTCamera(self.matrix @ t)
+
+    def difference(self, other):
+        """Relative transform world_to_cam(self) @ other (the reference
+        multiplies elementwise here, camera.py:571-582, which is not a
+        transform; the matrix product is what its docstring describes)."""
+        return RigidTransform(self.world_to_cam @ other.matrix)
+
+    @property
+    def center(self):
+        return (self.matrix[0, 3].item(), self.matrix[1, 3].item(),
+                self.matrix[2, 3].item())
+
+    def clone(self):
+        return RTCamera(self.matrix.clone())
+
+    def __str__(self):
+        return f"RTCamera({self.matrix.tolist()})"
+
+    __repr__ = __str__
+
+
+def quaternion_to_matrix(qw, qx, qy, qz):
+    """[3, 3] float64 rotation of the quaternion w + xi + yj + zk (normalised
+    first), R = I + 2w[v]x + 2[v]x^2 with v = (x, y, z)."""
+    q = torch.tensor([qw, qx, qy, qz], dtype=torch.double)
+    w, x, y, z = (q / q.norm()).tolist()
+    return torch.tensor([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]],
+        dtype=torch.double)
 
 
 class Project(torch.autograd.Function):

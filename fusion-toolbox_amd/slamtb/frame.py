@@ -27,6 +27,29 @@ class FrameInfo:
         self.rt_cam = rt_cam
         self.timestamp = timestamp
 
+    @classmethod
+    def from_json(cls, json):
+        """FTB JSON form (frame.py:51-76)."""
+        from .camera import KCamera, RTCamera
+        kcam = json.get('kcam', None)
+        rgb_kcam = json.get('rgb_kcam', None)
+        rt_cam = json.get('rt_cam', None)
+        return cls(KCamera.from_json(kcam) if kcam is not None else None,
+                   json.get('depth_scale', 1.0), json.get('depth_bias', 0.0),
+                   json.get('depth_max', 4500.0), json.get('timestamp', None),
+                   KCamera.from_json(rgb_kcam) if rgb_kcam is not None else None,
+                   RTCamera.from_json(rt_cam) if rt_cam is not None else None)
+
+    def to_json(self):
+        json = {}
+        for name, value in vars(self).items():
+            if value is None:
+                continue
+            if hasattr(value, 'to_json'):
+                value = value.to_json()
+            json[name] = value
+        return json
+
     def clone(self):
         return FrameInfo(self.kcam.clone() if self.kcam is not None else None,
                          self.depth_scale, self.depth_bias, self.depth_max,

@@ -1,7 +1,11 @@
 """Frame preprocessing for the ICP step (slamtb/processing.py of the reference):
 bilateral depth filter, normals, erosion, pyramid downsampling and the
-feature pyramid. Every function runs on the HIP device of its inputs.
+feature pyramid. Every function runs on the HIP device of its inputs,
+except the host-side colour conversion of :func:`to_color_feature`.
 """
+from enum import Enum
+
+import numpy as np
 import torch
 
 from . import _lib
@@ -131,7 +135,48 @@ def erode_mask(in_mask):
     return out_mask
 
 
+class ColorSpace(Enum):
+    """Colour spaces of :func:`to_color_feature` (processing.py:178-185)."""
+    RGB = 0
+    GRAY = 1
+    INTENSITY = 1
+    LAB = 2
+    HSV = 3
+
+
+def _box_blur5(img):
+    """5x5 mean with mirrored borders (gfedcb|abcdefgh|gfedcba), rounded
+    to the nearest integer as an 8-bit image filter stores it."""
+    pad = [(2, 2), (2, 2)] + [(0, 0)] * (img.ndim - 2)
+    x = np.pad(img.astype(np.int32), pad, mode="reflect")
+    H, W = img.shape[:2]
+    acc = np.zeros(img.shape, np.int32)
+    for dy in range(5):
+        for dx in range(5):
+            acc += x[dy:dy + H, dx:dx + W]
+    return np.floor(acc / 25.0 + 0.5).astype(np.uint8)
+
+
+def to_color_feature(rgb_image, color_space=ColorSpace.RGB, blur=False):
+    """RGB uint8 [H, W, 3] image -> float32 [C, H, W] features in [0, 1]
+    (processing.py:188-206): optional 5x5 box blur, optional conversion to
+    gray (0.299 R + 0.587 G + 0.114 B, rounded; C = 1), then /255 in
+    channel-first layout. The LAB and HSV conversions of the reference's
+    OpenCV call are not provided."""
+    img = np.asarray(rgb_image.cpu() if torch.is_tensor(rgb_image) else rgb_image)
+    if blur:
+        img = _box_blur5(img)
+    if color_space == ColorSpace.GRAY:
+        y = img[..., 0] * 0.299 + img[..., 1] * 0.587 + img[..., 2] * 0.114
+        img = np.floor(y + 0.5).clip(0, 255).astype(np.uint8)
+    elif color_space != ColorSpace.RGB:
+        raise NotImplementedError(f"to_color_feature: {color_space} is not provided")
+    t = torch.from_numpy(np.ascontiguousarray(img)).float().div_(255.0)
+    return t.unsqueeze(0) if t.dim() == 2 else t.permute(2, 0, 1).contiguous()
+
+
 __all__ = ["bilateral_depth_filter", "BilateralDepthFilter", "estimate_normals",
            "downsample_xyz", "downsample_mask", "downsample_features",
            "gaussian_blur3x3", "feature_pyramid", "erode_mask",
-           "EstimateNormalsMethod", "DownsampleXYZMethod"]
+           "EstimateNormalsMethod", "DownsampleXYZMethod", "ColorSpace",
+           "to_color_feature"]
