@@ -681,6 +681,22 @@ def ftb_section(dev):
         "rte_rmse_m": rmse(metrics.relative_translational_error(gt, pred)),
         "rre_rmse_rad": rmse(metrics.relative_rotational_error(gt, pred)),
         "ms_per_frame": el * 1e3 / (len(frames) - 1)}
+
+    # The tracking front end (SurfelSLAM.step without fusion): the default
+    # GN preset, eager and as one captured graph per step.
+    from slamtb.pipeline import SurfelSLAM
+    for mode in ("eager", "graph"):
+        slam = SurfelSLAM(device=dev, use_graph=mode == "graph")
+        stats = [slam.step(f) for f in frames]
+        stats += [slam.step(f) for f in frames[1:]]  # second pass: warm
+        times = sorted(st.ms for st in stats[len(frames):])
+        gtt = metrics.set_start_at_identity(
+            {i: f.info.rt_cam for i, f in enumerate(frames)})
+        pr = {i: st.rt_camera for i, st in enumerate(stats[:len(frames)])}
+        out[f"surfel_slam_tracking_{mode}"] = {
+            "ms_per_frame_median": times[len(times) // 2],
+            "ate_rmse_m": rmse(metrics.absolute_translational_error(gtt, pr)),
+            "accepted": sum(st.accepted for st in stats[1:len(frames)])}
     return out
 
 

@@ -12,10 +12,11 @@ def depth_image_to_uvz(depth_image, finfo):
     depth_image = depth_image.float() * finfo.depth_scale + finfo.depth_bias
     device = depth_image.device
     dtype = depth_image.dtype
-    ys, xs = torch.meshgrid(torch.arange(depth_image.size(0), dtype=dtype),
-                            torch.arange(depth_image.size(1), dtype=dtype),
-                            indexing="ij")
-    return torch.stack([xs.to(device), ys.to(device), depth_image], 2)
+    ys, xs = torch.meshgrid(
+        torch.arange(depth_image.size(0), dtype=dtype, device=device),
+        torch.arange(depth_image.size(1), dtype=dtype, device=device),
+        indexing="ij")
+    return torch.stack([xs, ys, depth_image], 2)
 
 
 def ensure_torch(x, dtype=None):
