@@ -33,6 +33,9 @@ synchronisations. Also reported:
     adjoint backward;
   * the differentiable frame matcher and projection kernels at 640x480 and
     AutogradICP driven by them, with the oracle's matcher beside them;
+  * point-cloud registration: surfel clouds of the pair voxel-merged at
+    1 cm, the exact neighbour search, the neighbour feature layer and
+    AutogradICP.estimate_pcl;
   * the reference's own RGB-D test frames: the Registration notebook's pair
     against the camera poses recorded with the frames, and frame-to-frame
     tracking over 15 frames;
@@ -72,6 +75,8 @@ def parse():
                     help="pairs of the batched workload (0 disables it)")
     ap.add_argument("--no-bf16", action="store_true",
                     help="skip the 1280x720 BF16-storage workload")
+    ap.add_argument("--no-pcl", action="store_true",
+                    help="skip the point-cloud registration section")
     ap.add_argument("--no-ftb", action="store_true",
                     help="skip the real-frame (FTB sample1) section")
     ap.add_argument("--no-matcher", action="store_true",
@@ -700,6 +705,88 @@ def ftb_section(dev):
     return out
 
 
+def pcl_section(pair, dev, with_cpu):
+    """Point-cloud registration at 640x480: both frames of the bench pair as
+    surfel clouds, voxel-merged at 1 cm, then the exact 8-nearest-neighbour
+    search within 0.1 m (uniform grid, and brute force beside it), the
+    nearest-neighbour feature layer forward and
+    backward, and AutogradICP.estimate_pcl; the oracle's neighbour search
+    on a 2,000-query sample beside it."""
+    import numpy as np
+    import torch
+    from slamtb import metrics
+    from slamtb._cslamtb import NearestNeighborsOp, knn
+    from slamtb.camera import KCamera
+    from slamtb.frame import Frame, FrameInfo, FramePointCloud
+    from slamtb.registration.autogradicp import AutogradICP
+    from slamtb.surfel import SurfelCloud
+
+    kc = KCamera(torch.from_numpy(pair["kcam"]))
+    clouds = []
+    for side in ("src", "tgt"):
+        d = torch.from_numpy(pair[f"{side}_depth"]).to(dev)
+        fp = FramePointCloud.from_frame(Frame(FrameInfo(kc), depth_image=d), device=dev)
+        clouds.append(SurfelCloud.from_frame_pcl(
+            fp, features=torch.from_numpy(pair[f"{side}_feats"]).to(dev)))
+    src_full, tgt_full = clouds
+    tgt_full.downsample(0.01)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        tgt_full.downsample(0.01)
+    torch.cuda.synchronize()
+    t_ds = (time.perf_counter() - t0) * 1e3 / 5  # host-synchronising: wall clock
+    src, tgt = src_full.downsample(0.01), tgt_full.downsample(0.01)
+    n, m = src.size, tgt.size
+    from slamtb._cslamtb import KnnGrid
+    KnnGrid(tgt.points, 0.1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grid = KnnGrid(tgt.points, 0.1)
+    torch.cuda.synchronize()
+    t_build = (time.perf_counter() - t0) * 1e3  # host-synchronising: wall clock
+    t_knn = launch_ms(lambda: grid.query(src.points, 8, 0.1), 20)
+    t_brute = launch_ms(lambda: knn(tgt.points, src.points, 8, 0.1, method="brute"), 3)
+    dist, index = grid.query(src.points, 8, 0.1)
+    C = tgt.feature_size
+    out = torch.empty(C, n, device=dev)
+    eps = torch.empty(n, 6, 8, device=dev)
+    dl = torch.randn(C, n, device=dev)
+    dx = torch.empty(n, 3, device=dev)
+    t_fwd = launch_ms(lambda: NearestNeighborsOp.forward(dist, index, tgt.features, out), 50)
+    t_eps = launch_ms(lambda: NearestNeighborsOp.compute_epsilon_distances(
+        tgt.points, src.points, index, eps), 50)
+    t_bwd = launch_ms(lambda: NearestNeighborsOp.backward(eps, index, tgt.features,
+                                                           dl, dx), 50)
+    icp = AutogradICP(20, learning_rate=0.1, geom_weight=1.0, feat_weight=0.0,
+                      distance_threshold=0.1, gradient_descent_iters=20)
+    icp.estimate_pcl(src, tgt)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = icp.estimate_pcl(src, tgt)
+    el = time.perf_counter() - t0
+    out_d = {"surfels_full": [src_full.size, tgt_full.size],
+             "surfels_1cm": [n, m],
+             "launch_ms": {"voxel_downsample_1cm": t_ds, "knn_grid_build": t_build,
+                           "knn_k8_0.1m_grid": t_knn, "knn_k8_0.1m_brute": t_brute,
+                           "nn_features_forward": t_fwd,
+                           "nn_epsilon_distances": t_eps,
+                           "nn_features_backward": t_bwd},
+             "knn_brute_pairs_per_s": n * m / (t_brute * 1e-3),
+             "estimate_pcl_s": el,
+             "estimate_pcl_pose_err_vs_truth": [float(e) for e in metrics.pose_error(
+                 res.transform.double().numpy(), pair["T_gt"])]}
+    if with_cpu:
+        sys.path.insert(0, str(ROOT))
+        from oracle import oracle as orc
+        q = src.points[:2000].cpu().numpy()
+        tg = tgt.points.cpu().numpy()
+        t0 = time.perf_counter()
+        orc.knn(tg, q, 8, 0.1)
+        out_d["cpu_oracle_knn_pairs_per_s"] = len(q) * m / (time.perf_counter() - t0)
+    return out_d
+
+
 def main():
     args = parse()
     if args.copy_probe > 0:
@@ -714,7 +801,7 @@ def main():
     from slamtb.registration._fused import runner_for
     if args.profile:
         args.no_oracle = args.no_cpu_baseline = args.no_bf16 = True
-        args.no_diff = args.no_matcher = args.no_ftb = True
+        args.no_diff = args.no_matcher = args.no_ftb = args.no_pcl = True
         args.pairs = 0
 
     dev = torch.device("cuda:0")
@@ -833,6 +920,9 @@ def main():
     if not args.no_ftb:
         log("reference RGB-D frames (FTB sample1)")
         out["ftb_sample1"] = ftb_section(dev)
+    if not args.no_pcl:
+        log("point-cloud registration at 640x480")
+        out["pcl_640x480"] = pcl_section(pair, dev, not args.no_oracle)
     if not args.no_matcher:
         log("differentiable matcher and AutogradICP at 640x480")
         out["matcher_640x480"] = matcher_section(pair, top, dev, not args.no_oracle)

@@ -12,6 +12,7 @@ output convention as the pybind11 bindings:
 * ``FPCLMatcherOp`` -- src/matching.cpp:8-9
 * ``ProjectOp`` -- src/camera.cpp:11-12
 * ``RigidTransformOp`` -- src/camera.cpp:40-45
+* ``NearestNeighborsOp`` -- src/nearest_neighbors.cpp:6-12
 
 Differences, all stricter or documented: tensors must live on a HIP device
 (the reference also had a CPU instantiation; this build has no CPU path),
@@ -19,6 +20,7 @@ output tensors must be contiguous, and ``kcam``/``rt_cam`` are cast to the
 source dtype instead of failing the accessor type check.
 """
 import enum
+import math
 
 import torch
 
@@ -513,7 +515,161 @@ class RigidTransformOp:
         raise NotImplementedError("RigidTransformOp.rodrigues is not provided")
 
 
+NN_DELTA_H = 0.0005000000237487257  # float(0.0005f), the reference's step
+
+
+class NearestNeighborsOp:
+    @staticmethod
+    def forward(nn_distances, nn_index, features, out_features):
+        """src/nearest_neighbors.cu:56-79: out_features [C, N] = neighbour
+        features averaged with normalised weights 1 / (1 + d)."""
+        what = "NearestNeighborsOp.forward"
+        require_gpu(nn_distances, nn_index, features, out_features, what=what)
+        require_contiguous_out(out_features, what=what)
+        d = _in(nn_distances)
+        dt = d.dtype
+        idx = _in(nn_index, torch.int64)
+        f = _in(features, dt)
+        n, k = d.shape
+        C, m = f.shape
+        if tuple(idx.shape) != (n, k):
+            raise RuntimeError(f"{what}: nn_index must be [{n}, {k}]")
+        if tuple(out_features.shape) != (C, n) or out_features.dtype != dt:
+            raise RuntimeError(f"{what}: out_features must be [{C}, {n}] {dt}")
+        with torch.cuda.device(d.device):
+            check(lib().stb_nn_features_forward(
+                dtype_code(d), ptr(d), ptr(idx), n, k, ptr(f), C, m,
+                ptr(out_features), stream()), what)
+
+    @staticmethod
+    def compute_epsilon_distances(target_xyz, source_xyz, nn_index,
+                                  epsilon_distances):
+        """src/nearest_neighbors.cu:145-164: epsilon_distances [N, 6, k]."""
+        what = "NearestNeighborsOp.compute_epsilon_distances"
+        require_gpu(target_xyz, source_xyz, nn_index, epsilon_distances, what=what)
+        require_contiguous_out(epsilon_distances, what=what)
+        dt = target_xyz.dtype
+        t = _in(target_xyz, dt).view(-1, 3)
+        sx = _in(source_xyz, dt).view(-1, 3)
+        idx = _in(nn_index, torch.int64)
+        n, k = idx.shape
+        if tuple(epsilon_distances.shape) != (n, 6, k) or epsilon_distances.dtype != dt:
+            raise RuntimeError(f"{what}: epsilon_distances must be [{n}, 6, {k}] {dt}")
+        with torch.cuda.device(t.device):
+            check(lib().stb_nn_epsilon_distances(
+                dtype_code(t), ptr(t), t.size(0), ptr(sx), ptr(idx), n, k,
+                NN_DELTA_H, ptr(epsilon_distances), stream()), what)
+
+    @staticmethod
+    def backward(epsilon_distances, nn_index, features, dl_features, dl_xyz):
+        """src/nearest_neighbors.cu:265-293: dl_xyz [N, 3]."""
+        what = "NearestNeighborsOp.backward"
+        require_gpu(epsilon_distances, nn_index, features, dl_features, dl_xyz,
+                    what=what)
+        require_contiguous_out(dl_xyz, what=what)
+        e = _in(epsilon_distances)
+        dt = e.dtype
+        idx = _in(nn_index, torch.int64)
+        n, k = idx.shape
+        f = _in(features, dt)
+        C, m = f.shape
+        dl = _in(dl_features, dt)
+        if tuple(dl.shape) != (C, n) or tuple(dl_xyz.shape) != (n, 3):
+            raise RuntimeError(f"{what}: expected dl_features [{C}, {n}], dl_xyz [{n}, 3]")
+        with torch.cuda.device(e.device):
+            check(lib().stb_nn_features_backward(
+                dtype_code(e), ptr(e), ptr(idx), n, k, ptr(f), C, m, ptr(dl),
+                NN_DELTA_H, ptr(dl_xyz), stream()), what)
+
+
+class KnnGrid:
+    """Uniform grid over target points for the bounded neighbour search:
+    cells at least ``cell`` wide (one cell of slack on each axis), the
+    targets stably sorted by cell, and each cell's start. Built on the
+    device; the bounding box is read once on the host."""
+
+    MAX_CELLS = 1 << 24
+
+    def __init__(self, target, cell):
+        what = "KnnGrid"
+        require_gpu(target, what=what)
+        t = _in(target).view(-1, 3)
+        self.dtype = t.dtype
+        self.m = t.size(0)
+        finite = torch.isfinite(t).all(1)
+        pts = t[finite] if not bool(finite.all()) else t
+        if pts.size(0) == 0:
+            lo, hi = [0.0] * 3, [0.0] * 3
+        else:
+            lo, hi = pts.min(0)[0].tolist(), pts.max(0)[0].tolist()
+        cell = float(cell) * (1 + 1e-4)
+        dims = [int((b - a) / cell) + 2 for a, b in zip(lo, hi)]
+        total = dims[0] * dims[1] * dims[2]
+        if total > self.MAX_CELLS:
+            cell *= (total / self.MAX_CELLS) ** (1 / 3) * 1.01
+            dims = [int((b - a) / cell) + 2 for a, b in zip(lo, hi)]
+        self.lo, self.cell, self.dims = lo, cell, dims
+        ncell = dims[0] * dims[1] * dims[2]
+        cells = torch.empty(self.m, dtype=torch.int32, device=t.device)
+        with torch.cuda.device(t.device):
+            check(lib().stb_grid_cells(dtype_code(t), ptr(t), self.m, lo[0], lo[1],
+                                       lo[2], cell, dims[0], dims[1], dims[2],
+                                       ptr(cells), stream()), what)
+        sorted_cells, order = torch.sort(cells, stable=True)
+        if t.dtype == torch.float32:
+            tag = order.to(torch.int32).view(torch.float32)
+        else:
+            tag = order.to(t.dtype)
+        self.entries = torch.cat([t[order], tag[:, None]], 1).contiguous()
+        self.start = torch.searchsorted(
+            sorted_cells, torch.arange(ncell + 1, dtype=torch.int32,
+                                       device=t.device)).to(torch.int64).contiguous()
+
+    def query(self, query, k, distance_upper_bound):
+        what = "knn"
+        q = _in(query, self.dtype).view(-1, 3)
+        n = q.size(0)
+        d = torch.empty(n, k, dtype=q.dtype, device=q.device)
+        i = torch.empty(n, k, dtype=torch.int64, device=q.device)
+        nx, ny, nz = self.dims
+        with torch.cuda.device(q.device):
+            check(lib().stb_knn_grid(
+                dtype_code(q), ptr(self.entries), ptr(self.start),
+                nx, ny, nz, self.lo[0], self.lo[1], self.lo[2], self.cell, self.m,
+                ptr(q), n, int(k), float(distance_upper_bound), ptr(d), ptr(i),
+                stream()), what)
+        return d, i
+
+
+def knn(target, query, k, distance_upper_bound=float("inf"), grid=None,
+        method="auto"):
+    """Exact k nearest neighbours on the device: (squared distances [n, k],
+    index [n, k] int64), (inf, m) in missing slots. Not a reference binding:
+    it stands in for the host k-d tree query of KDTreeLayer. A finite bound
+    searches a uniform grid (``grid``, a ``KnnGrid`` built with a cell at
+    least the bound, or one built here); an infinite bound, or
+    ``method="brute"``, searches all targets."""
+    what = "knn"
+    require_gpu(target, query, what=what)
+    q = _in(query).view(-1, 3)
+    t = _in(target, q.dtype).view(-1, 3)
+    n = q.size(0)
+    if method not in ("auto", "brute"):
+        raise ValueError(f"{what}: method must be 'auto' or 'brute'")
+    if method == "auto" and math.isfinite(distance_upper_bound) and t.size(0) > 0:
+        if grid is None or grid.cell < distance_upper_bound or grid.dtype != q.dtype:
+            grid = KnnGrid(t, distance_upper_bound)
+        return grid.query(q, k, distance_upper_bound)
+    d = torch.empty(n, k, dtype=q.dtype, device=q.device)
+    i = torch.empty(n, k, dtype=torch.int64, device=q.device)
+    with torch.cuda.device(q.device):
+        check(lib().stb_knn(dtype_code(q), ptr(t), t.size(0), ptr(q), n, int(k),
+                            float(distance_upper_bound), ptr(d), ptr(i), stream()),
+              what)
+    return d, i
+
+
 __all__ = ["CorrespondenceMap", "ICPJacobian", "Processing",
            "EstimateNormalsMethod", "DownsampleXYZMethod", "ExpRtToMatrixOp",
            "MatrixToExpRtOp", "FPCLMatcherOp", "ProjectOp", "RigidTransformOp",
-           "_lib"]
+           "NearestNeighborsOp", "knn", "KnnGrid", "_lib"]

@@ -129,6 +129,24 @@ _SIGNATURES = {
     "stb_rigid_transform_points": (ctypes.c_int, [_i32, _vp, _vp, _i64, _vp, _vp]),
     "stb_rigid_transform_normals": (ctypes.c_int, [_i32, _vp, _vp, _i64, _vp, _vp]),
     "stb_rigid_backward_workspace_bytes": (_sz, []),
+    "stb_knn": (ctypes.c_int, [_i32, _vp, _i64, _vp, _i64, _i32, _f64, _vp, _vp, _vp]),
+    "stb_grid_cells": (ctypes.c_int, [
+        _i32, _vp, _i64, _f64, _f64, _f64, _f64, _i32, _i32, _i32, _vp, _vp]),
+    "stb_knn_grid": (ctypes.c_int, [
+        _i32, _vp, _vp, _i32, _i32, _i32, _f64, _f64, _f64, _f64, _i64, _vp,
+        _i64, _i32, _f64, _vp, _vp, _vp]),
+    "stb_nn_features_forward": (ctypes.c_int, [
+        _i32, _vp, _vp, _i64, _i32, _vp, _i32, _i64, _vp, _vp]),
+    "stb_nn_epsilon_distances": (ctypes.c_int, [
+        _i32, _vp, _i64, _vp, _vp, _i64, _i32, _f64, _vp, _vp]),
+    "stb_nn_features_backward": (ctypes.c_int, [
+        _i32, _vp, _vp, _i64, _i32, _vp, _i32, _i64, _vp, _f64, _vp, _vp]),
+    "stb_voxel_ids": (ctypes.c_int, [
+        _vp, _i64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+        _i32, _i32, _vp, _vp]),
+    "stb_voxel_merge": (ctypes.c_int, [
+        _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _i64, _vp, _vp, _vp,
+        _vp, _vp, _vp, _vp, _vp]),
     "stb_rigid_transform_backward": (ctypes.c_int, [
         _i32, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _sz, _vp]),
     "stb_icp_workspace_bytes": (_sz, [_i64, _i32, _i32]),

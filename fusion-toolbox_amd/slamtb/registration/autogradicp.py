@@ -83,7 +83,7 @@ def _compacted_loss(tsrc, tpoints, tnormals, tfeatures, smask, source_feats,
         diff = tpoints - tsrc[smask]
         cost = (tnormals * diff).sum(1)
         geom_loss = torch.pow(cost, 2).mean()
-    if icp.feat_weight > 0 and source_feats is not None:
+    if icp.feat_weight > 0 and source_feats is not None and tfeatures is not None:
         feat_diff = torch.norm(tfeatures - source_feats[:, smask], 2, dim=0)
         if icp.feat_residual_thresh > 0:
             thr2 = icp.feat_residual_thresh * icp.feat_residual_thresh
@@ -252,11 +252,17 @@ class AutogradICP:
 
     def estimate_pcl(self, source_pcl, target_pcl, transform=None,
                      source_feats=None, target_feats=None):
-        """autogradicp.py:297-330 registers unorganised point clouds through
-        a k-d tree matcher, which is not provided here."""
-        raise NotImplementedError(
-            "AutogradICP.estimate_pcl needs the k-d tree matcher, which is "
-            "not provided; use estimate or estimate_frame")
+        """Align two point (or surfel) clouds (autogradicp.py:297-330):
+        8 nearest neighbours within ``distance_threshold`` per source point,
+        the nearest one's point and normal for the geometric term and their
+        interpolated features for the feature term."""
+        from slamtb.spatial.matching import PointCloudMatcher
+        matcher = PointCloudMatcher(
+            target_pcl.points, target_pcl.normals, target_feats,
+            num_neighbors=8, distance_upper_bound=self.distance_threshold,
+            normals_angle_thresh=self.normals_angle_thresh)
+        return self._estimate(source_pcl.points, source_pcl.normals,
+                              source_feats, matcher, transform)
 
     def estimate_frame(self, source_frame, target_frame, source_feats=None,
                        target_feats=None, transform=None, device="cpu"):

@@ -82,6 +82,26 @@ class MultiscaleRegistration:
         return curr_result
 
     def estimate_pcl(self, source_pcl, target_pcl, transform=None, device=None):
-        """Unordered point-cloud registration (multiscale.py:69-91) is outside
-        the frame-to-frame ICP step this package provides."""
-        raise NotImplementedError("estimate_pcl is not provided")
+        """Coarse-to-fine registration of point or surfel clouds
+        (multiscale.py:69-91): per estimator, both clouds voxel-merged at
+        its scale (a voxel size; 0 keeps the cloud), then the estimator's
+        ``estimate_pcl`` from the previous level's transform. The
+        estimators must provide ``estimate_pcl`` (``AutogradICP`` does; the
+        reference's ``ICPOdometry`` has none). Voxel merging runs on the
+        device instead of the reference's host round trip."""
+        if device is None:
+            device = source_pcl.device
+        if transform is not None:
+            transform = transform.cpu()
+        result = None
+        for scale, estimator in self.estimators[::-1]:
+            if scale > 0:
+                src = source_pcl.to(device).downsample(scale)
+                tgt = target_pcl.to(device).downsample(scale)
+            else:
+                src, tgt = source_pcl, target_pcl
+            result = estimator.estimate_pcl(src, tgt, transform=transform,
+                                            source_feats=src.features,
+                                            target_feats=tgt.features)
+            transform = result.transform
+        return result

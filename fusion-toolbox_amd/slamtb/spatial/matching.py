@@ -127,11 +127,47 @@ class FramePointCloudMatcher:
 
 
 class PointCloudMatcher:
-    """matching.py:127-172: k-d tree matching against an unorganised point
-    cloud. It needs the reference's KDTreeLayer (a scipy cKDTree on the
-    host), which is outside the frame-to-frame path built here."""
+    """Correspondences against an unorganised point cloud through the
+    nearest-neighbour layer (matching.py:127-172): the nearest target point
+    and normal of every source point with a neighbour within
+    ``distance_upper_bound``, and the target features interpolated from its
+    ``num_neighbors`` nearest neighbours (differentiable in the source
+    points). The reference's normal-angle filter is disabled code
+    (matching.py:160-170) and is not applied here either."""
 
-    def __init__(self, *args, **kwargs):
-        raise NotImplementedError(
-            "PointCloudMatcher (k-d tree matching) is not provided; use "
-            "FramePointCloudMatcher")
+    def __init__(self, target_points, target_normals, target_features,
+                 num_neighbors=5, distance_upper_bound=math.inf,
+                 normals_angle_thresh=1):
+        from .kdtree_layer import KDTreeLayer
+        self.target_points = target_points
+        self.target_normals = target_normals
+        self.target_features = target_features
+        self.num_neighbors = num_neighbors
+        self.distance_upper_bound = distance_upper_bound
+        self.normals_angle_thresh = normals_angle_thresh
+        self._layer = KDTreeLayer
+        self.kdtree_op = KDTreeLayer.setup(target_points)
+
+    @classmethod
+    def from_point_cloud(cls, pcl, features, num_neighbors=5,
+                         distance_upper_bound=math.inf):
+        return cls(pcl.points, pcl.normals, features, num_neighbors=num_neighbors,
+                   distance_upper_bound=distance_upper_bound)
+
+    def find_correspondences(self, source_points, source_normals=None):
+        """Returns (points [M, 3], normals [M, 3], features [C, M] or None,
+        mask [N]) of the M source points with a neighbour."""
+        # The layer's state is per class, as in the reference: re-register
+        # this matcher's target before querying.
+        self.kdtree_op = self._layer.setup(self.target_points)
+        match_mask = self._layer.query(
+            source_points, k=self.num_neighbors,
+            distance_upper_bound=self.distance_upper_bound)
+        top1 = self._layer.last_query[1][match_mask, 0]
+        matched_points = self.target_points[top1]
+        matched_normals = self.target_normals[top1]
+        matched_features = None
+        if self.target_features is not None:
+            matched_features = self.kdtree_op(self.target_features,
+                                              source_points)[:, match_mask]
+        return matched_points, matched_normals, matched_features, match_mask

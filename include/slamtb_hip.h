@@ -228,6 +228,89 @@ int stb_rigid_transform_normals(int32_t dtype, const void* matrix,
  * dx_points[n,3] = R^T dy (skipped when NULL) and d_matrix[3,4] =
  * sum_i dy_i [p_i; 1]^T, summed in FP64 in a fixed order. workspace is device
  * memory of at least stb_rigid_backward_workspace_bytes() bytes. */
+/* Exact k-nearest-neighbour search, the device replacement of the host
+ * k-d tree query of KDTreeLayer.query (slamtb/spatial/kdtree_layer.py:40-60,
+ * pykdtree KDTree.query with sqr_dists=True): for each of n query points the
+ * k (1..16) nearest of m target points ([m,3] / [n,3], dtype STB_F32 or
+ * STB_F64) with squared distance strictly below distance_upper_bound^2
+ * (pass +inf for no bound), sorted by squared distance and then by target
+ * index; out_sq_dist [n,k] and out_index [n,k] hold (inf, m) in unused
+ * slots. Exact brute force (the reference's k-d tree searched approximately,
+ * eps = 0.5). */
+int stb_knn(int32_t dtype, const void* target, int64_t m, const void* query,
+            int64_t n, int32_t k, double distance_upper_bound,
+            void* out_sq_dist, int64_t* out_index, void* stream);
+
+/* Uniform grid for stb_knn_grid: the cell of each of m points, (z * ny + y)
+ * * nx + x with x = floor((p.x - lo_x) / cell) etc., or nx * ny * nz for a
+ * point outside the grid or not finite. */
+int stb_grid_cells(int32_t dtype, const void* points, int64_t m, double lo_x,
+                   double lo_y, double lo_z, double cell, int32_t nx, int32_t ny,
+                   int32_t nz, int32_t* out_cells, void* stream);
+
+/* stb_knn with a finite distance bound over a uniform grid (cell >= bound):
+ * entries [m,4] holds the targets stably sorted by cell as (x, y, z, index),
+ * the index's int32 bits in the fourth float for STB_F32 and its value for
+ * STB_F64; cell_start [nx*ny*nz + 1] the first sorted position of each cell.
+ * Same result as stb_knn (ties broken by original index). */
+int stb_knn_grid(int32_t dtype, const void* entries, const int64_t* cell_start,
+                 int32_t nx, int32_t ny, int32_t nz,
+                 double lo_x, double lo_y, double lo_z, double cell, int64_t m,
+                 const void* query, int64_t n, int32_t k,
+                 double distance_upper_bound, void* out_sq_dist,
+                 int64_t* out_index, void* stream);
+
+/* NearestNeighborsOp.forward (src/nearest_neighbors.cpp:7,
+ * src/nearest_neighbors.cu:13-79): out_features [channels, n] = sum over the
+ * neighbours of features[:, index] * w, w = (1 / (1 + d)) / sum of 1 / (1 + d)
+ * over the finite distances; features is [channels, m]. */
+int stb_nn_features_forward(int32_t dtype, const void* nn_distances,
+                            const int64_t* nn_index, int64_t n, int32_t k,
+                            const void* features, int32_t channels, int64_t m,
+                            void* out_features, void* stream);
+
+/* NearestNeighborsOp.compute_epsilon_distances (src/nearest_neighbors.cpp:9,
+ * src/nearest_neighbors.cu:86-164): epsilon_distances [n, 6, k], the
+ * normalised weights 1 / (1 + |target - (source +- h e_axis)|^2) of each
+ * query's neighbours for the six offsets (h = 0.0005 in the reference, as a
+ * float). */
+int stb_nn_epsilon_distances(int32_t dtype, const void* target_xyz, int64_t m,
+                             const void* source_xyz, const int64_t* nn_index,
+                             int64_t n, int32_t k, double h,
+                             void* epsilon_distances, void* stream);
+
+/* NearestNeighborsOp.backward (src/nearest_neighbors.cpp:8,
+ * src/nearest_neighbors.cu:166-295): dl_xyz [n,3] = sum over channels of the
+ * central difference (step h) of the weighted feature average times
+ * dl_features [channels, n]. */
+int stb_nn_features_backward(int32_t dtype, const void* epsilon_distances,
+                             const int64_t* nn_index, int64_t n, int32_t k,
+                             const void* features, int32_t channels, int64_t m,
+                             const void* dl_features, double h, void* dl_xyz,
+                             void* stream);
+
+/* Voxel id of each point (HashVolume::Hash, include/slamtb/hash_volume.hpp:
+ * 67-77): int((p - min) / voxel_size) per axis, combined as
+ * depth * width * height + row * width + col in int arithmetic, with width
+ * and height the truncated extents of the volume. */
+int stb_voxel_ids(const float* points, int64_t n, float min_x, float min_y,
+                  float min_z, float voxel_size, int32_t width, int32_t height,
+                  int32_t* out_ids, void* stream);
+
+/* SurfelVolume.merge + to_surfel_cloud (src/surfel_volume.cpp:13-14,
+ * src/surfel_volume.cu:12-123): per voxel the mean of its members' points,
+ * normals, colours (truncated to uint8), confidences, radii and features
+ * [channels, n]; times are 0, as the reference leaves them. `order` lists the
+ * point indices stably sorted by voxel id and `starts` [num_voxels + 1] the
+ * start of each voxel's run in it. */
+int stb_voxel_merge(const float* points, const float* normals,
+                    const uint8_t* colors, const float* confidences,
+                    const float* radii, const float* features, int32_t channels,
+                    int64_t n, const int64_t* order, const int64_t* starts,
+                    int64_t num_voxels, float* out_points, float* out_normals,
+                    uint8_t* out_colors, float* out_confidences, float* out_radii,
+                    int32_t* out_times, float* out_features, void* stream);
+
 size_t stb_rigid_backward_workspace_bytes(void);
 int stb_rigid_transform_backward(int32_t dtype, const void* matrix,
                                  const void* points, int64_t n, const void* dy,

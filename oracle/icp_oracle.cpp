@@ -34,11 +34,16 @@
 //   rigid_points / rigid_backward: pinned against matrix products, the
 //     inverse-transpose of random matrices and finite differences
 //     (test_oracle_matching.py);
+//   knn / nn_forward / nn_epsilon / nn_backward / voxel_merge: pinned by
+//     numpy brute force (sorted distances), closed forms (equidistant
+//     neighbours average, a linear feature field's exact difference
+//     quotient) and hand voxel layouts (test_oracle_pcl.py);
 //   fpcl_forward / fpcl_backward / project_*: pinned by closed forms
 //     (features linear in u and v, whose bilinear sample and central
 //     difference are exact), a brute-force matcher over the map, and finite
 //     differences of the projection (test_oracle_matching.py).
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -515,6 +520,176 @@ void rigid_backward(const T* m, const T* p, const T* dy, int64_t n, T* dx,
     }
   }
   for (int k = 0; k < 12; ++k) dM[k] = T(acc[k]);
+}
+
+// Exact k nearest neighbours by brute force: squared distance dx*dx + dy*dy +
+// dz*dz (left to right), strictly below bound2, ordered by distance then by
+// target index; missing slots (inf, m) as the k-d tree of
+// slamtb/spatial/kdtree_layer.py:40-60 reports them.
+template <typename T>
+void knn(const T* tgt, int64_t m, const T* qry, int64_t n, int k, T bound2,
+         T* out_d, int64_t* out_i) {
+  std::vector<std::pair<T, int64_t>> best;
+  for (int64_t q = 0; q < n; ++q) {
+    best.clear();
+    const V3<T> p = load3(qry, q);
+    for (int64_t j = 0; j < m; ++j) {
+      const T dx = p.x - tgt[3 * j], dy = p.y - tgt[3 * j + 1],
+              dz = p.z - tgt[3 * j + 2];
+      const T d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < bound2) best.emplace_back(d2, j);
+    }
+    std::stable_sort(best.begin(), best.end(),
+                     [](const std::pair<T, int64_t>& a,
+                        const std::pair<T, int64_t>& b) { return a.first < b.first; });
+    for (int s = 0; s < k; ++s) {
+      const bool have = s < int(best.size());
+      out_d[q * k + s] = have ? best[s].first : std::numeric_limits<T>::infinity();
+      out_i[q * k + s] = have ? best[s].second : m;
+    }
+  }
+}
+
+// NearestNeighborsOp (src/nearest_neighbors.cu): ForwardKernel :13-54,
+// EpsilonDistancesKernel :86-143, NNGrad + BackwardKernel :166-262.
+template <typename T>
+void nn_forward(const T* dist, const int64_t* index, int64_t n, int k,
+                const T* feats, int C, int64_t m, T* out) {
+  for (int64_t q = 0; q < n; ++q) {
+    T total = T(0);
+    for (int s = 0; s < k; ++s)
+      if (dist[q * k + s] < std::numeric_limits<T>::infinity())
+        total += T(1) / (T(1) + dist[q * k + s]);
+    for (int c = 0; c < C; ++c) {
+      T f = T(0);
+      for (int s = 0; s < k; ++s) {
+        const int64_t j = index[q * k + s];
+        if (j >= m) break;
+        const T sim = T(1) / (T(1) + dist[q * k + s]);
+        const T w = sim / total;
+        f += feats[c * m + j] * w;
+      }
+      out[c * n + q] = f;
+    }
+  }
+}
+
+template <typename T>
+void nn_epsilon(const T* tgt, int64_t m, const T* src, const int64_t* index,
+                int64_t n, int k, T h, T* eps) {
+  for (int64_t row = 0; row < n; ++row) {
+    for (int col = 0; col < 6; ++col) {
+      T e[3] = {T(0), T(0), T(0)};
+      e[col / 2] = (col % 2 == 0) ? h : -h;
+      const V3<T> s = load3(src, row);
+      const V3<T> x{e[0] + s.x, e[1] + s.y, e[2] + s.z};
+      T* out = eps + (row * 6 + col) * k;
+      T total = T(0);
+      int found = 0;
+      for (int j = 0; j < k; ++j) {
+        const int64_t t = index[row * k + j];
+        if (t > m - 1) break;
+        const V3<T> d = sub(load3(tgt, t), x);
+        const T sim = T(1) / (T(1) + dot3(d, d));
+        out[j] = sim;
+        total += sim;
+        ++found;
+      }
+      if (found == 0) {
+        out[0] = T(1);
+        continue;
+      }
+      for (int j = 0; j < found; ++j) out[j] = out[j] / total;
+    }
+  }
+}
+
+template <typename T>
+void nn_backward(const T* eps, const int64_t* index, int64_t n, int k,
+                 const T* feats, int C, int64_t m, const T* dl, T h, T* dx) {
+  const T div = T(1) / (T(2) * h);
+  for (int64_t q = 0; q < n; ++q) {
+    T g3[3] = {T(0), T(0), T(0)};
+    for (int c = 0; c < C; ++c) {
+      T avg[6];
+      for (int w = 0; w < 6; ++w) {
+        T a = T(0);
+        for (int s = 0; s < k; ++s) {
+          const int64_t j = index[q * k + s];
+          if (j < m) a += feats[c * m + j] * eps[(q * 6 + w) * k + s];
+        }
+        avg[w] = a;
+      }
+      const T g = dl[c * n + q];
+      for (int a = 0; a < 3; ++a) g3[a] += (avg[2 * a] - avg[2 * a + 1]) * div * g;
+    }
+    for (int a = 0; a < 3; ++a) dx[3 * q + a] = g3[a];
+  }
+}
+
+// SurfelVolume::Merge + ToSurfelCloud (src/surfel_volume.cu:12-123) with
+// HashVolume::Hash (include/slamtb/hash_volume.hpp:20-77): sequential sums
+// per voxel in point order, scaled by 1 / count; voxels in increasing id.
+// Returns the number of voxels; outputs must hold n entries.
+int64_t voxel_merge(const float* pts, const float* nrm, const uint8_t* col,
+                    const float* conf, const float* rad, const float* feats,
+                    int C, int64_t n, float vs, float* o_pts, float* o_nrm,
+                    uint8_t* o_col, float* o_conf, float* o_rad,
+                    int32_t* o_time, float* o_feats) {
+  if (n == 0) return 0;
+  float mn[3], mx[3];
+  for (int d = 0; d < 3; ++d) mn[d] = mx[d] = pts[d];
+  for (int64_t i = 1; i < n; ++i)
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = std::min(mn[d], pts[3 * i + d]);
+      mx[d] = std::max(mx[d], pts[3 * i + d]);
+    }
+  const int width = int((mx[0] - mn[0]) / vs);
+  const int height = int((mx[1] - mn[1]) / vs);
+  std::vector<std::pair<int32_t, int64_t>> ids(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const int c = int((pts[3 * i] - mn[0]) / vs);
+    const int r = int((pts[3 * i + 1] - mn[1]) / vs);
+    const int d = int((pts[3 * i + 2] - mn[2]) / vs);
+    ids[i] = {d * width * height + r * width + c, i};
+  }
+  std::stable_sort(ids.begin(), ids.end(),
+                   [](const std::pair<int32_t, int64_t>& a,
+                      const std::pair<int32_t, int64_t>& b) { return a.first < b.first; });
+  std::vector<int64_t> starts;
+  for (int64_t a = 0; a < n; ++a)
+    if (a == 0 || ids[a].first != ids[a - 1].first) starts.push_back(a);
+  const int64_t V = int64_t(starts.size());
+  starts.push_back(n);
+  for (int64_t v = 0; v < V; ++v) {
+    float p[3] = {0, 0, 0}, q[3] = {0, 0, 0}, cc[3] = {0, 0, 0}, cf = 0, r = 0;
+    for (int64_t a = starts[v]; a < starts[v + 1]; ++a) {
+      const int64_t i = ids[a].second;
+      for (int d = 0; d < 3; ++d) {
+        p[d] += pts[3 * i + d];
+        q[d] += nrm[3 * i + d];
+        cc[d] += float(col[3 * i + d]);
+      }
+      cf += conf[i];
+      r += rad[i];
+    }
+    const float inv = 1.0f / float(starts[v + 1] - starts[v]);
+    for (int d = 0; d < 3; ++d) {
+      o_pts[3 * v + d] = p[d] * inv;
+      o_nrm[3 * v + d] = q[d] * inv;
+      o_col[3 * v + d] = uint8_t(cc[d] * inv);
+    }
+    o_conf[v] = cf * inv;
+    o_rad[v] = r * inv;
+    o_time[v] = 0;
+    for (int ch = 0; ch < C; ++ch) {
+      float f = 0;
+      for (int64_t a = starts[v]; a < starts[v + 1]; ++a)
+        f += feats[ch * n + ids[a].second];
+      o_feats[ch * n + v] = f * inv;
+    }
+  }
+  return V;
 }
 
 // InformationMatrixKernel::operator() (src/icp_jacobian_geometric.cu:117-158):
@@ -1080,5 +1255,38 @@ ORC_FPCL(f64, double)
   }
 ORC_RIGID(f32, float)
 ORC_RIGID(f64, double)
+
+#define ORC_PCL(SFX, T)                                                        \
+  void orc_knn_##SFX(const T* tgt, int64_t m, const T* qry, int64_t n, int k,  \
+                     double bound, T* d, int64_t* i) {                        \
+    const T b2 = std::isfinite(bound) ? T(bound) * T(bound)                   \
+                                      : std::numeric_limits<T>::infinity();   \
+    knn<T>(tgt, m, qry, n, k, b2, d, i);                                      \
+  }                                                                           \
+  void orc_nn_forward_##SFX(const T* d, const int64_t* i, int64_t n, int k,   \
+                            const T* f, int C, int64_t m, T* out) {           \
+    nn_forward<T>(d, i, n, k, f, C, m, out);                                  \
+  }                                                                           \
+  void orc_nn_epsilon_##SFX(const T* tgt, int64_t m, const T* src,            \
+                            const int64_t* i, int64_t n, int k, double h,     \
+                            T* e) {                                           \
+    nn_epsilon<T>(tgt, m, src, i, n, k, T(h), e);                             \
+  }                                                                           \
+  void orc_nn_backward_##SFX(const T* e, const int64_t* i, int64_t n, int k,  \
+                             const T* f, int C, int64_t m, const T* dl,       \
+                             double h, T* dx) {                               \
+    nn_backward<T>(e, i, n, k, f, C, m, dl, T(h), dx);                        \
+  }
+ORC_PCL(f32, float)
+ORC_PCL(f64, double)
+
+int64_t orc_voxel_merge(const float* pts, const float* nrm, const uint8_t* col,
+                        const float* conf, const float* rad, const float* feats,
+                        int C, int64_t n, float vs, float* o_pts, float* o_nrm,
+                        uint8_t* o_col, float* o_conf, float* o_rad,
+                        int32_t* o_time, float* o_feats) {
+  return voxel_merge(pts, nrm, col, conf, rad, feats, C, n, vs, o_pts, o_nrm,
+                     o_col, o_conf, o_rad, o_time, o_feats);
+}
 
 }  // extern "C"

@@ -273,6 +273,85 @@ def rigid_backward(matrix, points, dy):
     return dx, dM
 
 
+NN_DELTA_H = float(np.float32(0.0005))  # the reference's float constant
+
+
+def knn(target, query, k, distance_upper_bound=np.inf):
+    """Exact k nearest neighbours: (squared distances [n,k], index [n,k]),
+    missing slots (inf, m)."""
+    dt = np.asarray(query).dtype
+    t = _c(np.asarray(target).reshape(-1, 3), dt)
+    q = _c(np.asarray(query).reshape(-1, 3), dt)
+    d = np.empty((q.shape[0], k), dt)
+    i = np.empty((q.shape[0], k), np.int64)
+    getattr(lib(), f"orc_knn_{_sfx(dt)}")(
+        _p(t), ctypes.c_int64(t.shape[0]), _p(q), ctypes.c_int64(q.shape[0]),
+        ctypes.c_int(k), ctypes.c_double(distance_upper_bound), _p(d), _p(i))
+    return d, i
+
+
+def nn_forward(dist, index, features):
+    dt = np.asarray(dist).dtype
+    d = _c(dist, dt)
+    n, k = d.shape
+    f = _c(features, dt)
+    C, m = f.shape
+    out = np.empty((C, n), dt)
+    getattr(lib(), f"orc_nn_forward_{_sfx(dt)}")(
+        _p(d), _p(_c(index, np.int64)), ctypes.c_int64(n), ctypes.c_int(k),
+        _p(f), ctypes.c_int(C), ctypes.c_int64(m), _p(out))
+    return out
+
+
+def nn_epsilon(target, source, index, h=NN_DELTA_H):
+    dt = np.asarray(source).dtype
+    t = _c(np.asarray(target).reshape(-1, 3), dt)
+    s = _c(np.asarray(source).reshape(-1, 3), dt)
+    idx = _c(index, np.int64)
+    n, k = idx.shape
+    eps = np.zeros((n, 6, k), dt)
+    getattr(lib(), f"orc_nn_epsilon_{_sfx(dt)}")(
+        _p(t), ctypes.c_int64(t.shape[0]), _p(s), _p(idx), ctypes.c_int64(n),
+        ctypes.c_int(k), ctypes.c_double(h), _p(eps))
+    return eps
+
+
+def nn_backward(eps, index, features, dl, h=NN_DELTA_H):
+    dt = np.asarray(eps).dtype
+    idx = _c(index, np.int64)
+    n, k = idx.shape
+    f = _c(features, dt)
+    C, m = f.shape
+    dx = np.empty((n, 3), dt)
+    getattr(lib(), f"orc_nn_backward_{_sfx(dt)}")(
+        _p(_c(eps, dt)), _p(idx), ctypes.c_int64(n), ctypes.c_int(k), _p(f),
+        ctypes.c_int(C), ctypes.c_int64(m), _p(_c(dl, dt)), ctypes.c_double(h),
+        _p(dx))
+    return dx
+
+
+def voxel_merge(points, normals, colors, confidences, radii, features, voxel_size):
+    """SurfelVolume merge: dict of per-voxel means, voxels in increasing id."""
+    n = len(points)
+    f = None if features is None else _c(features, np.float32)
+    C = 0 if f is None else f.shape[0]
+    out = dict(points=np.empty((n, 3), np.float32), normals=np.empty((n, 3), np.float32),
+               colors=np.empty((n, 3), np.uint8), confidences=np.empty(n, np.float32),
+               radii=np.empty(n, np.float32), times=np.empty(n, np.int32),
+               features=np.empty((C, max(n, 1)), np.float32))
+    lib().orc_voxel_merge.restype = ctypes.c_int64
+    V = lib().orc_voxel_merge(
+        _p(_c(points, np.float32)), _p(_c(normals, np.float32)), _p(_c(colors, np.uint8)),
+        _p(_c(confidences, np.float32)), _p(_c(radii, np.float32)),
+        _p(f) if f is not None else ctypes.c_void_p(0), ctypes.c_int(C),
+        ctypes.c_int64(n), ctypes.c_float(voxel_size), _p(out["points"]),
+        _p(out["normals"]), _p(out["colors"]), _p(out["confidences"]),
+        _p(out["radii"]), _p(out["times"]), _p(out["features"]))
+    res = {key: v[:V] for key, v in out.items() if key != "features"}
+    res["features"] = out["features"][:, :V]
+    return res
+
+
 def exp_rt_to_matrix(x):
     x = np.asarray(x)
     dt = x.dtype if x.dtype in (np.float32, np.float64) else np.float64
