@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-template <typename S, bool GEOM, bool FEAT>
+template <typename S, bool GEOM, bool FEAT, int NB>
 __global__ void __launch_bounds__(kBlock)
     batch_jacobian_kernel(const void* __restrict__ plan,
                           stb_icp_state* __restrict__ states,
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
     for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
       const int64_t idx = base + k;
-      if (idx < L.n) jacobian_pixel<S, GEOM, FEAT>(L, m, map, idx, tw, acc);
+      if (idx < L.n) jacobian_pixel<S, GEOM, FEAT, NB>(L, m, map, idx, tw, acc);
     }
     block_partial(acc, partials + int64_t(t) * kSlots, sh, false);
   }
@@ -204,13 +204,19 @@ int grid_tiles(const stb_icp_params* P, int tiles, int dflt) {
 }
 
 template <typename S, bool GEOM, bool FEAT>
-void launch_batch_jacobian(const void* plan, int num_pairs,
+void launch_batch_jacobian(const void* plan, int num_pairs, int max_channels,
                            stb_icp_state* states, const Scratch& sc, int grid,
                            const TermWeights& tw, int do_solve, int fixed,
                            hipStream_t s) {
-  hipLaunchKernelGGL((batch_jacobian_kernel<S, GEOM, FEAT>), dim3(grid),
-                     dim3(kBlock), 0, s, plan, states, sc.map0, sc.map1,
-                     sc.partials, tw, fixed);
+  // Batches of two feature channels when no pair has four besides its last.
+  if (FEAT && max_channels - 1 < 4)
+    hipLaunchKernelGGL((batch_jacobian_kernel<S, GEOM, FEAT, 2>), dim3(grid),
+                       dim3(kBlock), 0, s, plan, states, sc.map0, sc.map1,
+                       sc.partials, tw, fixed);
+  else
+    hipLaunchKernelGGL((batch_jacobian_kernel<S, GEOM, FEAT, 4>), dim3(grid),
+                       dim3(kBlock), 0, s, plan, states, sc.map0, sc.map1,
+                       sc.partials, tw, fixed);
   hipLaunchKernelGGL(batch_solve_kernel, dim3(num_pairs), dim3(kSolveBlock), 0,
                      s, plan, states, sc.partials, tw.gw, tw.fw, do_solve,
                      fixed);
@@ -249,13 +255,13 @@ void launch_batch(const void* plan, const stb_icp_batch_sizes& z,
     }
     const int solve = phase == kIterate ? 1 : 0;
     if (P.use_geom && P.use_feat)
-      launch_batch_jacobian<S, true, true>(plan, z.num_pairs, states, sc, gj,
+      launch_batch_jacobian<S, true, true>(plan, z.num_pairs, z.max_channels, states, sc, gj,
                                            tw, solve, fixed, s);
     else if (P.use_geom)
-      launch_batch_jacobian<S, true, false>(plan, z.num_pairs, states, sc, gj,
+      launch_batch_jacobian<S, true, false>(plan, z.num_pairs, z.max_channels, states, sc, gj,
                                             tw, solve, fixed, s);
     else
-      launch_batch_jacobian<S, false, true>(plan, z.num_pairs, states, sc, gj,
+      launch_batch_jacobian<S, false, true>(plan, z.num_pairs, z.max_channels, states, sc, gj,
                                             tw, solve, fixed, s);
   }
 }
@@ -321,8 +327,10 @@ extern "C" int stb_icp_batch_plan(int32_t num_pairs,
   std::vector<int32_t> start(size_t(num_pairs) + 1, 0);
   std::vector<int64_t> off(size_t(num_pairs), 0);
   int64_t tiles = 0, maps = 0;
+  int32_t max_channels = 0;
   for (int p = 0; p < num_pairs; ++p) {
     const stb_icp_level& L = levels[p];
+    if (L.channels > max_channels) max_channels = L.channels;
     STB_REQUIRE(L.src_points && L.src_normals && L.src_mask && L.tgt_points &&
                     L.tgt_normals && L.tgt_mask,
                 "stb_icp_batch_plan: pair %d has a NULL array", p);
@@ -346,6 +354,8 @@ extern "C" int stb_icp_batch_plan(int32_t num_pairs,
   sizes->total_map = maps;
   sizes->plan_bytes = l.bytes;
   sizes->scratch_bytes = sc.bytes;
+  sizes->max_channels = max_channels;
+  sizes->reserved = 0;
   if (plan_host == nullptr) return STB_OK;  // size query
   STB_REQUIRE(plan_capacity >= l.bytes, "stb_icp_batch_plan: buffer too small");
   char* b = static_cast<char*>(plan_host);

@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(kBlock)
 // cap): the slot accumulators are then born from that pixel's terms instead
 // of living across a loop (~84 VGPRs instead of ~116). Larger levels use the
 // grid-stride loop.
-template <typename S, bool GEOM, bool FEAT, bool ONE_PX>
+template <typename S, bool GEOM, bool FEAT, bool ONE_PX, int NB>
 __global__ void __launch_bounds__(kBlock)
     jacobian_fused_kernel(stb_icp_level Ld, stb_icp_state* __restrict__ st,
                           unsigned long long* __restrict__ map0,
@@ -72,10 +72,10 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
   if (ONE_PX) {
-    if (gtid < L.n) jacobian_pixel<S, GEOM, FEAT>(L, m, map, gtid, tw, acc);
+    if (gtid < L.n) jacobian_pixel<S, GEOM, FEAT, NB>(L, m, map, gtid, tw, acc);
   } else {
     for (int64_t idx = gtid; idx < L.n; idx += gsize)
-      jacobian_pixel<S, GEOM, FEAT>(L, m, map, idx, tw, acc);
+      jacobian_pixel<S, GEOM, FEAT, NB>(L, m, map, idx, tw, acc);
   }
   if (!fixed_map) {  // clear the other map for the next iteration
     unsigned long long* __restrict__ next = pick_map(st, map0, map1, 1);
@@ -247,20 +247,33 @@ int check_level(const stb_icp_level* L, bool need_feats) {
 // the loop; kLinearize and kInformation are one-off passes on the spare map.
 enum Phase { kCorrespond, kUpdate, kLinearize, kInformation, kJacobians };
 
+template <typename S, bool GEOM, bool FEAT, int NB>
+void launch_jacobian_nb(const stb_icp_level& L, int grid, bool one_px,
+                        const Workspace& w, const TermWeights& tw,
+                        int fixed_map, hipStream_t stream) {
+  if (one_px)
+    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, true, NB>),
+                       dim3(grid), dim3(kBlock), 0, stream, L, w.st, w.map[0],
+                       w.map[1], w.map_len, w.partials, tw, fixed_map);
+  else
+    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, false, NB>),
+                       dim3(grid), dim3(kBlock), 0, stream, L, w.st, w.map[0],
+                       w.map[1], w.map_len, w.partials, tw, fixed_map);
+}
+
 template <typename S, bool GEOM, bool FEAT>
 void launch_jacobian(const stb_icp_level& L, const stb_icp_params& P,
                      const Workspace& w, const TermWeights& tw, int do_solve,
                      int fixed_map, hipStream_t stream) {
   bool one_px = false;
   const int grid = update_grid(P, L.n_src, &one_px);
-  if (one_px)
-    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, true>), dim3(grid),
-                       dim3(kBlock), 0, stream, L, w.st, w.map[0], w.map[1],
-                       w.map_len, w.partials, tw, fixed_map);
+  // Feature channels other than the last load in batches of four, or of two
+  // when there are fewer than four of them (three-channel colour: no
+  // padding slots).
+  if (FEAT && L.channels - 1 < 4)
+    launch_jacobian_nb<S, GEOM, FEAT, 2>(L, grid, one_px, w, tw, fixed_map, stream);
   else
-    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, false>),
-                       dim3(grid), dim3(kBlock), 0, stream, L, w.st, w.map[0],
-                       w.map[1], w.map_len, w.partials, tw, fixed_map);
+    launch_jacobian_nb<S, GEOM, FEAT, 4>(L, grid, one_px, w, tw, fixed_map, stream);
   if (do_solve < 0) return;  // the Jacobian launch alone
   const int groups = grid < kSolveGroups ? grid : kSolveGroups;
   hipLaunchKernelGGL(solve_fused_kernel, dim3(groups), dim3(kBlock), 0, stream,

@@ -150,7 +150,7 @@ __device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
 // loads issue in one batch: a pixel costs two memory round trips. A pixel
 // that does not own its target adds exact zeros (selected, never multiplied
 // in, so NaN or inf in its unused data cannot leak into the sums).
-template <typename S, bool GEOM, bool FEAT>
+template <typename S, bool GEOM, bool FEAT, int NB = 4>
 __device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
                                       const unsigned long long* __restrict__ map,
                                       int64_t idx, const TermWeights& tw,
@@ -185,7 +185,7 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
   }
   if (FEAT) {
     float jf[6], rl;
-    const float sq = feature_terms<float, S, S, true>(
+    const float sq = feature_terms<float, S, S, true, NB>(
         L.tf, L.C, L.H, L.W, L.HW, L.sf, L.n, idx, L.fx, L.fy, p, u, v, jf, rl);
     const bool use = own && !(sq > tw.fthr2);
     float j[6];

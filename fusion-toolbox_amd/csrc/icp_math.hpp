@@ -109,7 +109,7 @@ __device__ inline T bil_get(const S* __restrict__ ch, int H, int W,
 // reciprocal of z (about 1 ulp) instead of four correctly rounded divides;
 // the Jacobian values only enter sums checked to 1e-5, while the decisions
 // (projection, gates, owner) keep exact IEEE division everywhere.
-template <typename T, typename S, typename SF, bool FAST = false>
+template <typename T, typename S, typename SF, bool FAST = false, int NB = 4>
 __device__ inline T feature_terms(const S* __restrict__ tf, int C, int H, int W,
                                   int64_t HW, const SF* __restrict__ sf,
                                   int64_t n, int64_t idx, T fx, T fy,
@@ -129,22 +129,23 @@ __device__ inline T feature_terms(const S* __restrict__ tf, int C, int H, int W,
   const T su0 = bil_get(ch, H, W, bu0), su1 = bil_get(ch, H, W, bu1);
   const T sv0 = bil_get(ch, H, W, bv0), sv1 = bil_get(ch, H, W, bv1);
   const T flast = T(ldw(sf, int64_t(C - 1) * n + idx));
-  // Channels 0 .. C-2 in batches of four whose loads issue together; a
-  // batch slot past C-2 re-reads channel C-2 and adds +0, which leaves the
-  // running sum (and the reference's summation order) unchanged.
+  // Channels 0 .. C-2 in batches of NB (4, or 2 for few channels) whose
+  // loads issue together; a batch slot past C-2 re-reads channel C-2 and
+  // adds +0, which leaves the running sum (and the reference's summation
+  // order) unchanged.
   T sq = T(0);
   const int last_c = C - 2 > 0 ? C - 2 : 0;
-  for (int c0i = 0; c0i < C - 1; c0i += 4) {
-    // All sixteen taps and four source values are loaded before any is used.
-    T taps[4][4], fs[4];
+  for (int c0i = 0; c0i < C - 1; c0i += NB) {
+    // All the batch's taps and source values load before any is used.
+    T taps[NB][4], fs[NB];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NB; ++k) {
       const int c = c0i + k < last_c ? c0i + k : last_c;
       bil_taps(tf + c * HW, H, W, c0, taps[k]);
       fs[k] = T(ldw(sf, c * n + idx));
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NB; ++k) {
       const T res = bil_mix(H, W, c0, taps[k]) - fs[k];
       sq += (c0i + k < C - 1) ? res * res : T(0);
     }

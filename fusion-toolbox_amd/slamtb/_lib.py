@@ -71,7 +71,7 @@ class StbIcpBatchSizes(ctypes.Structure):
     _fields_ = [
         ("num_pairs", _i32), ("total_tiles", _i32), ("tile_px", _i32),
         ("storage", _i32), ("total_map", _i64), ("plan_bytes", _sz),
-        ("scratch_bytes", _sz),
+        ("scratch_bytes", _sz), ("max_channels", _i32), ("reserved", _i32),
     ]
 
 

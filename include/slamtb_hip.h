@@ -457,6 +457,8 @@ typedef struct stb_icp_batch_sizes {
   int64_t total_map;    /* sum of target pixels over the pairs */
   size_t plan_bytes;    /* size of the plan image */
   size_t scratch_bytes; /* device scratch: two maps per pair, partials, tickets */
+  int32_t max_channels; /* most feature channels of any pair */
+  int32_t reserved;
 } stb_icp_batch_sizes;
 
 /* Host only. Builds the plan image of one pyramid level for `num_pairs`
