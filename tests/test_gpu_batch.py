@@ -27,12 +27,13 @@ def registration():
                                    (0.5, ICPOdometry(3, **KW))])
 
 
-def make_pairs(sizes, seed0=40, storage=None):
+def make_pairs(sizes, seed0=40, storage=None, channels=3):
     ms = registration()
     pairs, truth = [], []
     for i, (W, H) in enumerate(sizes):
         k = synthetic.intrinsics(W, H)
-        pr = synthetic.make_pair(seed0 + i, W, H, kcam=k)
+        pr = synthetic.make_pair(seed0 + i, W, H, kcam=k, channels=channels,
+                                 descriptor=channels > 3)
         kcam = KCamera(torch.from_numpy(k))
         fps = []
         for side in ("src", "tgt"):
@@ -94,3 +95,18 @@ def test_batch_graph_replay_is_repeatable():
         again = [r.transform.numpy() for r in batch.results()]
         for a, b in zip(first, again):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("storage,channels", [(torch.bfloat16, 3), (None, 16),
+                                              (torch.float16, 16)])
+def test_batch_storage_and_channel_widths(storage, channels):
+    """16-bit storage and wide descriptors (feature batches of four) through
+    the batched kernels: equal to the single-pair path on the same inputs."""
+    sizes = [(160, 120), (240, 180), (80, 60)]
+    ms, pairs, _ = make_pairs(sizes, seed0=80, storage=storage, channels=channels)
+    results = BatchedMultiscaleICP(ms.estimators, pairs, DEV).estimate()
+    for i, levels in enumerate(pairs):
+        ref = single(ms, levels)
+        err = orc.pose_error(results[i].transform.numpy(), ref.transform.numpy())
+        assert err[0] < 1e-6 and err[1] < 1e-6, (i, err)
+        assert results[i].iterations == ref.iterations

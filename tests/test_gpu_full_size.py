@@ -3,9 +3,9 @@
 * 640x480, 3 levels (10/5/3 iterations), geometric + 3-channel features,
   bilateral-filtered depth: pose within 1e-6 of the oracle fed the same level
   inputs.
-* 1280x720 with 32-channel descriptors in BF16 storage (top level, 3
-  iterations): pose within 1e-6 of the oracle on the same (BF16-rounded)
-  values.
+* 1280x720 with 32-channel descriptors in BF16 and in FP16 storage (top
+  level, 3 iterations): pose within 1e-6 of the oracle on the same
+  (rounded) values.
 * 1280x720 fused linearisation against the drop-in per-pixel partials summed
   in FP64 on the host (the drop-in partials are pinned bit-exact to the
   oracle at 320x240): 1e-5 relative. No oracle run at this size.
@@ -92,15 +92,16 @@ def test_640x480_three_levels_match_the_oracle():
     assert res.iterations == 10
 
 
-def test_1280x720_bf16_descriptors_match_the_oracle():
+@pytest.mark.parametrize("storage", [torch.bfloat16, torch.float16])
+def test_1280x720_16bit_descriptors_match_the_oracle(storage):
     pair = synthetic.make_pair(502, 1280, 720, kcam=synthetic.intrinsics(1280, 720),
                                channels=32, descriptor=True)
     ms = MultiscaleRegistration([(1.0, ICPOdometry(3, **KW))])
-    levels = gpu_levels(ms, pair, storage=torch.bfloat16)
-    assert levels[0][1].src_points.dtype == torch.bfloat16
+    levels = gpu_levels(ms, pair, storage=storage)
+    assert levels[0][1].src_points.dtype == storage
     res = run_levels(levels)
     err = orc.pose_error(res.transform.numpy(), oracle_chain(levels))
-    print("1280x720 BF16 C=32 pose error vs oracle", err)
+    print(f"1280x720 {storage} C=32 pose error vs oracle", err)
     assert err[0] < 1e-6 and err[1] < 1e-6
 
 
