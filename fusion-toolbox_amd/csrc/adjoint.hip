@@ -69,6 +69,87 @@ __device__ inline void bil_scatter(float* __restrict__ ch, int W, const BilD& b,
   if (b.rx && b.ry) atomicAdd(ch + o + W + 1, g * b.ur * b.vr);
 }
 
+// The four tap weights of a sample times g, zero for taps past the
+// right/bottom edge (as bil_scatter would skip them).
+struct TapW {
+  float w00, w10, w01, w11;
+};
+__device__ inline TapW tap_weights(const BilD& b, float g) {
+  TapW t;
+  t.w00 = g * (1.0f - b.ur) * (1.0f - b.vr);
+  t.w10 = b.rx ? g * b.ur * (1.0f - b.vr) : 0.0f;
+  t.w01 = b.ry ? g * (1.0f - b.ur) * b.vr : 0.0f;
+  t.w11 = (b.rx && b.ry) ? g * b.ur * b.vr : 0.0f;
+  return t;
+}
+
+// Scatter of the five samples of the feature Jacobian (the centre (ui, vi)
+// cell and the +-h cells, each the centre's or its neighbour along one
+// axis) merged in registers over the 4x4 texel patch whose corner is
+// (ui - 1, vi - 1): one atomic per touched texel -- four when all five
+// samples share the centre cell -- instead of four per sample.
+__device__ inline void scatter5(float* __restrict__ ch, int W, const BilD& c0,
+                                float g0, const BilD& bu0, float gu0,
+                                const BilD& bu1, float gu1, const BilD& bv0,
+                                float gv0, const BilD& bv1, float gv1) {
+  float P[4][4];
+#pragma unroll
+  for (int y = 0; y < 4; ++y)
+#pragma unroll
+    for (int x = 0; x < 4; ++x) P[y][x] = 0.0f;
+  const TapW c = tap_weights(c0, g0);
+  P[1][1] += c.w00;
+  P[1][2] += c.w10;
+  P[2][1] += c.w01;
+  P[2][2] += c.w11;
+  // Samples displaced along x by +1 (bu0 may) or -1 (bu1 may).
+  const TapW a = tap_weights(bu0, gu0);
+  const bool ra = bu0.ui != c0.ui;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const float w0 = r ? a.w01 : a.w00, w1 = r ? a.w11 : a.w10;
+    P[1 + r][1] += ra ? 0.0f : w0;
+    P[1 + r][2] += ra ? w0 : w1;
+    P[1 + r][3] += ra ? w1 : 0.0f;
+  }
+  const TapW b = tap_weights(bu1, gu1);
+  const bool lb = bu1.ui != c0.ui;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const float w0 = r ? b.w01 : b.w00, w1 = r ? b.w11 : b.w10;
+    P[1 + r][0] += lb ? w0 : 0.0f;
+    P[1 + r][1] += lb ? w1 : w0;
+    P[1 + r][2] += lb ? 0.0f : w1;
+  }
+  // Samples displaced along y by +1 (bv0 may) or -1 (bv1 may).
+  const TapW d = tap_weights(bv0, gv0);
+  const bool dd = bv0.vi != c0.vi;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const float w0 = q ? d.w10 : d.w00, w1 = q ? d.w11 : d.w01;
+    P[1][1 + q] += dd ? 0.0f : w0;
+    P[2][1 + q] += dd ? w0 : w1;
+    P[3][1 + q] += dd ? w1 : 0.0f;
+  }
+  const TapW e = tap_weights(bv1, gv1);
+  const bool ue = bv1.vi != c0.vi;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const float w0 = q ? e.w10 : e.w00, w1 = q ? e.w11 : e.w01;
+    P[0][1 + q] += ue ? w0 : 0.0f;
+    P[1][1 + q] += ue ? w1 : w0;
+    P[2][1 + q] += ue ? 0.0f : w1;
+  }
+  const int64_t o = (int64_t(c0.vi) - 1) * W + (c0.ui - 1);
+#pragma unroll
+  for (int y = 0; y < 4; ++y)
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+      // Non-zero entries are inside the image (weights past any edge are
+      // zero), so only in-range offsets are dereferenced.
+      if (P[y][x] != 0.0f) atomicAdd(ch + (o + int64_t(y) * W + x), P[y][x]);
+}
+
 // Block sum of the 12 pose-gradient terms in FP64 (lanes by shuffles, waves
 // through LDS), written as one kSlots-wide partial (unused slots zero) for
 // the fixed-order sum of the last block.
@@ -212,11 +293,8 @@ __device__ inline void adjoint_pixel(const LevelView<float>& L, const float* m,
       const float vb = rb * c0.dv + dub * div * (bu0.dv - bu1.dv) +
                        dvb * div * (bv0.dv - mv * bv1.dv);
       float* gch = G.d_tgt_feats + int64_t(C - 1) * L.HW;
-      bil_scatter(gch, L.W, c0, rb);
-      bil_scatter(gch, L.W, bu0, dub * div);
-      bil_scatter(gch, L.W, bu1, -dub * div);
-      bil_scatter(gch, L.W, bv0, dvb * div);
-      bil_scatter(gch, L.W, bv1, -dvb * div);
+      scatter5(gch, L.W, c0, rb, bu0, dub * div, bu1, -dub * div, bv0,
+               dvb * div, bv1, -dvb * div);
       G.d_src_feats[int64_t(C - 1) * L.n + idx] -= rb;
       // u = fx x / z + cx, v = fy y / z + cy
       pb.x += ub * L.fx / z;

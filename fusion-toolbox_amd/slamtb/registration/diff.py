@@ -203,12 +203,17 @@ class _UnrolledGN(torch.autograd.Function):
         for k in reversed(range(len(Ts))):
             ok = acc[k]
             A = torch.where(ok[:, None, None], As[k], eye)
-            x = torch.linalg.solve(A, torch.where(ok[:, None], bs[k], 0.0))
+            # A is symmetric positive definite for every accepted step (its
+            # device Cholesky succeeded); one factorisation serves both
+            # solves, with no host synchronisation.
+            Lc = torch.linalg.cholesky_ex(A)[0]
+            x = torch.cholesky_solve(torch.where(ok[:, None], bs[k], 0.0)[..., None],
+                                     Lc)[..., 0]
             U = exp_rt(x)
             dU = G[:, :3] @ Ts[k].transpose(1, 2)
             G_direct = U.transpose(1, 2) @ G
             xbar = exp_rt_vjp(dU, x, U)
-            lam = torch.linalg.solve(A, xbar)
+            lam = torch.cholesky_solve(xbar[..., None], Lc)[..., 0]
             coef = torch.cat([lam, x], 1)
             coef = torch.where(ok[:, None], coef, 0.0).contiguous()
             bt.begin(Ts[k])
