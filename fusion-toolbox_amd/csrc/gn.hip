@@ -1,10 +1,12 @@
 // Fused single-pair Gauss-Newton path (ICPOdometry): per iteration a
-// correspondence kernel, a Jacobian kernel that leaves one FP64 partial per
-// block, and a one-block kernel that sums the partials in a fixed order,
-// solves the 6x6 system and updates the pose on the device. (A last-block
-// ticket inside the Jacobian launch would cost every block a device-scope
-// fence and an atomic on one shared counter, about 20 ns per block in all --
-// 10 us for 512 blocks, several times the cost of the third launch.)
+// correspondence kernel and a Jacobian kernel that leaves one FP64 partial
+// per block. The partials are summed in a fixed order, the 6x6 system solved
+// and the pose updated on the device, by the next iteration's correspondence
+// launch (every block redundantly, see SolveRecord) or, after the last
+// iteration and in the split entry points, by a small launch of its own. (A
+// last-block ticket inside the Jacobian launch would cost every block a
+// device-scope fence and an atomic on one shared counter, about 20 ns per
+// block in all -- 10 us for 512 blocks.)
 #include "gn_device.hpp"
 
 namespace stb {
@@ -44,72 +46,165 @@ __global__ void __launch_bounds__(kBlock)
     corresp_pixel(L, m, nm, idx, map, dthr2, cstar);
 }
 
+// The correspondence launch of iteration k > 0 of a loop: every block first
+// sums iteration k-1's `count` partials (sum_partials, the same order in
+// every block), solves and updates the pose in one wavefront, then builds
+// the map at the new pose. Block 0 leaves the outcome in `rec` for the
+// Jacobian launch of the same iteration, which commits it to the state.
+// Every load that does not depend on the new pose -- the thread's first
+// source pixel, the previous pose, the partials, the state flags -- issues
+// before the first wait, so the update costs one memory round trip plus the
+// factorisation; barriers do not wait for outstanding global loads.
+template <typename S>
+__global__ void __launch_bounds__(kSolveBlock)
+    corresp_solve_kernel(stb_icp_level Ld, const stb_icp_state* __restrict__ st,
+                         SolveRecord* __restrict__ rec,
+                         const double* __restrict__ partials, int count,
+                         unsigned long long* __restrict__ map0,
+                         unsigned long long* __restrict__ map1, float dthr2,
+                         float cstar) {
+#pragma clang fp contract(off)
+  __shared__ double sh[kSolveBlock];
+  __shared__ double tot[kSlots];
+  __shared__ double To[12], Tn[12];
+  __shared__ float rtf[12];
+  __shared__ int s_ok;
+  const int t = threadIdx.x;
+  const LevelView<S> L = view_of<S>(Ld);
+  // A contiguous run of pixels per block (see jacobian_fused_kernel).
+  const int64_t chunk = (L.n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = int64_t(blockIdx.x) * chunk;
+  const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
+  const int64_t idx = lo + t;
+  // Unconditional (index clamped) so that no branch waits for them.
+  const int64_t ic = idx < L.n ? idx : L.n - 1;
+  const uint8_t vs = L.sm[ic];
+  const V3<float> ps = ld3f(L.sp, ic);
+  const V3<float> ns = ld3f(L.sn, ic);
+  const double row = st->transform[t < 12 ? t : 0];
+  // The flags stopped, iterations, ticket, map_sel (consecutive words) in
+  // one vector load, lane l of each wavefront taking word l % 4: a uniform
+  // scalar load would be issued where first used, after the factorisation.
+  const int32_t flag = (&st->stopped)[t & 3];
+  sum_partials<kSolveBlock>(partials, count, sh, tot);
+  if (t < 12) To[t] = row;
+  __syncthreads();
+  if (t < 64) {
+    const bool good = wave_solve_update(tot, To, Tn, rtf);
+    if (t == 0) s_ok = good;
+  }
+  const int stopped = __shfl(flag, 0, 64);
+  const int iters = __shfl(flag, 1, 64);
+  const int sel = __shfl(flag, 3, 64);
+  __syncthreads();
+  const bool ok = !stopped && s_ok != 0;
+  Rigid<float> r;
+  if (ok) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) r.m[i] = rtf[i];
+    make_normal_matrix(r);
+  } else {  // stopped before, or stopping now: the pose stays
+#pragma unroll
+    for (int i = 0; i < 12; ++i) r.m[i] = st->rt[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.nm[i] = st->nm[i];
+  }
+  const int use_sel = ok ? (sel ^ 1) : sel;
+  if (blockIdx.x == 0) {
+    if (!stopped && t < kSlots) rec->tot[t] = tot[t];
+    if (ok && t < 12) rec->T[t] = Tn[t];
+    if (t == 0) {
+#pragma unroll
+      for (int i = 0; i < 12; ++i) rec->rt[i] = r.m[i];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) rec->nm[i] = r.nm[i];
+      rec->active = !stopped;
+      rec->ok = ok;
+      rec->map_sel = use_sel;
+      rec->iterations = iters + 1;
+    }
+  }
+  unsigned long long* __restrict__ map = use_sel ? map1 : map0;
+  if (idx < hi)
+    corresp_pixel_at(L, r.m, r.nm, idx, vs, ps, ns, map, dthr2, cstar);
+  for (int64_t i = idx + kSolveBlock; i < hi; i += kSolveBlock)
+    corresp_pixel(L, r.m, r.nm, i, map, dthr2, cstar);
+}
+
 // Per-pixel Jacobians of one iteration; every block leaves one FP64 partial
-// for solve_fused_kernel (the next launch), so no block waits on another.
-// With ONE_PX each thread owns exactly one pixel (levels that fit the grid
-// cap): the slot accumulators are then born from that pixel's terms instead
-// of living across a loop (~84 VGPRs instead of ~116). Larger levels use the
-// grid-stride loop.
-template <typename S, bool GEOM, bool FEAT, bool ONE_PX, int NB>
-__global__ void __launch_bounds__(kBlock)
+// for the launch that sums them, so no block waits on another. With ONE_PX
+// each thread owns exactly one pixel (levels that fit the grid cap): the
+// slot accumulators are then born from that pixel's terms instead of living
+// across a loop (~84 VGPRs instead of ~116). Larger levels use the
+// grid-stride loop. With `rec` (iterations after the first of a loop) the
+// pose and map come from the record of this iteration's correspondence
+// launch, and block 0 commits that record to the state.
+template <typename S, bool GEOM, bool FEAT, bool ONE_PX, int NB, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
     jacobian_fused_kernel(stb_icp_level Ld, stb_icp_state* __restrict__ st,
+                          const SolveRecord* __restrict__ rec,
                           unsigned long long* __restrict__ map0,
                           unsigned long long* __restrict__ map1,
                           int64_t map_len, double* __restrict__ partials,
                           TermWeights tw, int fixed_map) {
-  __shared__ double sh[kShared];
+  __shared__ double sh[(BLOCK / 64) * kSlots];
   // No early exit on state.stopped (see corresp_fused_kernel): the solve
-  // launch ignores the partials of a stopped loop.
-  const unsigned long long* __restrict__ map =
-      pick_map(st, map0, map1, fixed_map);
-  const int64_t gsize = int64_t(gridDim.x) * blockDim.x;
+  // ignores the partials of a stopped loop.
+  const int sel = rec ? rec->map_sel : st->map_sel;
+  const float* __restrict__ rt = rec ? rec->rt : st->rt;
+  unsigned long long* map = (sel ^ fixed_map) ? map1 : map0;
+  // A loop's map is emptied by its owners as they read it; one-off passes
+  // (fixed_map) leave theirs for repeated calls.
+  const bool clear = !fixed_map;
   const int64_t gtid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   float m[12];
 #pragma unroll
-  for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
+  for (int i = 0; i < 12; ++i) m[i] = rt[i];
   const LevelView<S> L = view_of<S>(Ld);
   float acc[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
   if (ONE_PX) {
-    if (gtid < L.n) jacobian_pixel<S, GEOM, FEAT, NB>(L, m, map, gtid, tw, acc);
+    if (gtid < L.n)
+      jacobian_pixel<S, GEOM, FEAT, NB>(L, m, map, gtid, tw, acc, clear);
   } else {
-    for (int64_t idx = gtid; idx < L.n; idx += gsize)
-      jacobian_pixel<S, GEOM, FEAT, NB>(L, m, map, idx, tw, acc);
+    // Each block takes a contiguous run of pixels, so the pixels beyond one
+    // per thread spread over every block instead of doubling the work of the
+    // first ones (with one 1024-thread block per CU, a grid-stride loop left
+    // 44 CUs with twice the work at 640x480).
+    const int64_t chunk = (L.n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = int64_t(blockIdx.x) * chunk;
+    const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
+    for (int64_t idx = lo + threadIdx.x; idx < hi; idx += BLOCK)
+      jacobian_pixel<S, GEOM, FEAT, NB>(L, m, map, idx, tw, acc, clear);
   }
-  if (!fixed_map) {  // clear the other map for the next iteration
-    unsigned long long* __restrict__ next = pick_map(st, map0, map1, 1);
-    for (int64_t i = gtid; i < map_len; i += gsize) next[i] = kKeySentinel;
+  if (!fixed_map && rec == nullptr) {
+    // The first iteration of a loop (or a split update) also sweeps the
+    // other map, which a one-off pass may have left set: from here on both
+    // maps are empty at the start of every correspondence launch.
+    unsigned long long* __restrict__ next = sel ? map0 : map1;
+    const int64_t chunk = (map_len + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = int64_t(blockIdx.x) * chunk;
+    const int64_t hi = lo + chunk < map_len ? lo + chunk : map_len;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += BLOCK) next[i] = kKeySentinel;
   }
-  block_partial(acc, partials + int64_t(blockIdx.x) * kSlots, sh, false);
+  block_partial<BLOCK>(acc, partials + int64_t(blockIdx.x) * kSlots, sh, false);
+  // Last, so that its round trip does not delay block 0's pixels.
+  if (rec && blockIdx.x == 0) commit_record(st, rec, tw.gw, tw.fw);
 }
 
-// The fixed-order sum of the Jacobian launch's `count` partials, then the
-// pose update (do_solve) or the stored linearisation. kSolveGroups blocks
-// each sum a contiguous range of partials into a group partial; the last to
-// arrive adds the groups in order and finishes -- a handful of fences and
-// tickets instead of one per Jacobian block, and many CUs pulling partials.
-__global__ void __launch_bounds__(kBlock)
+// The fixed-order sum of the Jacobian launch's `count` partials by one
+// block -- the order every correspondence launch that applies an update uses
+// too, so a loop and the split entry points give the same bits -- then the
+// pose update (do_solve) or the stored linearisation.
+__global__ void __launch_bounds__(kSolveBlock)
     solve_fused_kernel(stb_icp_state* __restrict__ st,
                        const double* __restrict__ partials, int count,
-                       double* __restrict__ groups, double gw, double fw,
-                       int do_solve, int fixed_map) {
-  __shared__ double sh[kShared];
+                       double gw, double fw, int do_solve, int fixed_map) {
+  __shared__ double sh[kSolveBlock];
   __shared__ double tot[kSlots];
-  __shared__ int s_last;
   if (!fixed_map && st->stopped) return;
-  const int G = int(gridDim.x), g = int(blockIdx.x);
-  const int lo = int(int64_t(count) * g / G);
-  const int hi = int(int64_t(count) * (g + 1) / G);
-  sum_partials<kBlock>(partials + int64_t(lo) * kSlots, hi - lo, sh, tot);
-  if (threadIdx.x < kSlots) {
-    groups[g * kSlots + threadIdx.x] = tot[threadIdx.x];
-    __threadfence();
-  }
-  __syncthreads();
-  if (!arrive_last(&st->ticket, G - 1, &s_last)) return;
-  if (threadIdx.x == 0) st->ticket = 0;
-  finish_partials<kBlock>(st, groups, G, do_solve, gw, fw, sh, tot);
+  finish_partials<kSolveBlock>(st, partials, count, do_solve, gw, fw, sh, tot);
 }
 
 template <typename S>
@@ -166,20 +261,29 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------------------
-// Workspace layout: [state | map0 | map1 | partials]
+// Workspace layout: [state | map0 | map1 | partials | record]
 // ---------------------------------------------------------------------------
 
-constexpr int kDefaultUpdateBlocks = 1024;
+// Jacobian launch shape. By default at most kDefaultUpdateBlocks partials,
+// so that every block of the next correspondence launch sums them (64 KB)
+// cheaply: levels above kWidePixels use blocks of kSolveBlock threads (at
+// 640x480, 256 blocks of 1024 threads, the same occupancy as 1024 blocks of
+// 256), smaller ones blocks of kBlock. An explicit grid_update keeps blocks
+// of kBlock with that cap.
+constexpr int kDefaultUpdateBlocks = 256;
+constexpr int64_t kWidePixels = 2 * 65536;
 constexpr int kMaxUpdateBlocks = 4096;
 constexpr int kInformationBlocks = 256;
-constexpr int kSolveGroups = 16;
 constexpr int kDefaultCorrespBlocks = 4096;
+// Blocks of the correspondence launch that also applies the previous update
+// (kSolveBlock threads each; every block sums the partials): one per CU.
+constexpr int kCorrespSolveBlocks = 256;
 
 struct Workspace {
   stb_icp_state* st;
   unsigned long long* map[2];
   double* partials;
-  double* groups;
+  SolveRecord* rec;
   int64_t map_len;
 };
 
@@ -187,7 +291,7 @@ size_t ws_bytes(int32_t H, int32_t W) {
   const size_t map_len = size_t(H) * size_t(W);
   return align256(sizeof(stb_icp_state)) + 2 * align256(map_len * 8) +
          align256(size_t(kMaxUpdateBlocks) * kSlots * sizeof(double)) +
-         align256(size_t(kSolveGroups) * kSlots * sizeof(double));
+         align256(sizeof(SolveRecord));
 }
 
 Workspace carve(void* base, int32_t H, int32_t W) {
@@ -202,26 +306,43 @@ Workspace carve(void* base, int32_t H, int32_t W) {
   p += align256(size_t(w.map_len) * 8);
   w.partials = reinterpret_cast<double*>(p);
   p += align256(size_t(kMaxUpdateBlocks) * kSlots * sizeof(double));
-  w.groups = reinterpret_cast<double*>(p);
+  w.rec = reinterpret_cast<SolveRecord*>(p);
   return w;
 }
 
-// Jacobian grid: one pixel per thread when that fits the cap (grid_update,
-// or kDefaultUpdateBlocks), else `cap` blocks looping over pixels. The
-// kernel is bound by vector-instruction issue once all its loads are in
-// flight, and at 640x480 1024 looping blocks beat 1200 one-pixel blocks
-// (7.3 against 7.9 us): each block's reduction is shared by more pixels.
-int update_grid(const stb_icp_params& P, int64_t n, bool* one_px) {
-  int cap = P.grid_update > 0 ? P.grid_update : kDefaultUpdateBlocks;
-  if (cap > kMaxUpdateBlocks) cap = kMaxUpdateBlocks;
-  const int g = grid_for(n);
-  *one_px = g <= cap;
-  return g > cap ? cap : g;
+// Jacobian grid: one pixel per thread when that fits the cap, else `cap`
+// blocks looping over pixels. The kernel is bound by vector-instruction
+// issue once all its loads are in flight, and at 640x480 looping blocks beat
+// one-pixel blocks (7.3 against 7.9 us for 1024 against 1200 blocks of
+// 256): each block's reduction is shared by more pixels.
+struct JacShape {
+  int block;
+  int grid;
+  bool one_px;
+};
+
+JacShape jac_shape(const stb_icp_params& P, int64_t n) {
+  JacShape s;
+  int cap = kDefaultUpdateBlocks;
+  s.block = kBlock;
+  if (P.grid_update > 0)
+    cap = P.grid_update > kMaxUpdateBlocks ? kMaxUpdateBlocks : P.grid_update;
+  else if (n > kWidePixels)
+    s.block = kSolveBlock;
+  const int g = grid_for(n, s.block);
+  s.one_px = g <= cap && s.block == kBlock;
+  s.grid = g > cap ? cap : g;
+  return s;
 }
 
 int corresp_grid(const stb_icp_params& P, int64_t n) {
   const int cap = P.grid_corresp > 0 ? P.grid_corresp : kDefaultCorrespBlocks;
   return grid_for(n, kBlock, cap);
+}
+
+int corresp_solve_grid(const stb_icp_params& P, int64_t n) {
+  const int cap = P.grid_corresp > 0 ? P.grid_corresp : kCorrespSolveBlocks;
+  return grid_for(n, kSolveBlock, cap);
 }
 
 int check_level(const stb_icp_level* L, bool need_feats) {
@@ -248,37 +369,60 @@ int check_level(const stb_icp_level* L, bool need_feats) {
 enum Phase { kCorrespond, kUpdate, kLinearize, kInformation, kJacobians };
 
 template <typename S, bool GEOM, bool FEAT, int NB>
-void launch_jacobian_nb(const stb_icp_level& L, int grid, bool one_px,
-                        const Workspace& w, const TermWeights& tw,
-                        int fixed_map, hipStream_t stream) {
-  if (one_px)
-    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, true, NB>),
-                       dim3(grid), dim3(kBlock), 0, stream, L, w.st, w.map[0],
-                       w.map[1], w.map_len, w.partials, tw, fixed_map);
+void launch_jacobian_nb(const stb_icp_level& L, const JacShape& js,
+                        const Workspace& w, const SolveRecord* rec,
+                        const TermWeights& tw, int fixed_map,
+                        hipStream_t stream) {
+  if (js.block == kSolveBlock)
+    hipLaunchKernelGGL(
+        (jacobian_fused_kernel<S, GEOM, FEAT, false, NB, kSolveBlock>),
+        dim3(js.grid), dim3(kSolveBlock), 0, stream, L, w.st, rec, w.map[0],
+        w.map[1], w.map_len, w.partials, tw, fixed_map);
+  else if (js.one_px)
+    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, true, NB, kBlock>),
+                       dim3(js.grid), dim3(kBlock), 0, stream, L, w.st, rec,
+                       w.map[0], w.map[1], w.map_len, w.partials, tw,
+                       fixed_map);
   else
-    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, false, NB>),
-                       dim3(grid), dim3(kBlock), 0, stream, L, w.st, w.map[0],
-                       w.map[1], w.map_len, w.partials, tw, fixed_map);
+    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, false, NB, kBlock>),
+                       dim3(js.grid), dim3(kBlock), 0, stream, L, w.st, rec,
+                       w.map[0], w.map[1], w.map_len, w.partials, tw,
+                       fixed_map);
 }
 
+// The Jacobian launch, then (do_solve >= 0) the launch that sums its
+// partials and applies them: a pose update (1) or the stored linearisation
+// (0). do_solve < 0: the Jacobian launch alone.
 template <typename S, bool GEOM, bool FEAT>
 void launch_jacobian(const stb_icp_level& L, const stb_icp_params& P,
-                     const Workspace& w, const TermWeights& tw, int do_solve,
-                     int fixed_map, hipStream_t stream) {
-  bool one_px = false;
-  const int grid = update_grid(P, L.n_src, &one_px);
+                     const Workspace& w, const SolveRecord* rec,
+                     const TermWeights& tw, int do_solve, int fixed_map,
+                     hipStream_t stream) {
+  const JacShape js = jac_shape(P, L.n_src);
   // Feature channels other than the last load in batches of four, or of two
   // when there are fewer than four of them (three-channel colour: no
   // padding slots).
   if (FEAT && L.channels - 1 < 4)
-    launch_jacobian_nb<S, GEOM, FEAT, 2>(L, grid, one_px, w, tw, fixed_map, stream);
+    launch_jacobian_nb<S, GEOM, FEAT, 2>(L, js, w, rec, tw, fixed_map, stream);
   else
-    launch_jacobian_nb<S, GEOM, FEAT, 4>(L, grid, one_px, w, tw, fixed_map, stream);
-  if (do_solve < 0) return;  // the Jacobian launch alone
-  const int groups = grid < kSolveGroups ? grid : kSolveGroups;
-  hipLaunchKernelGGL(solve_fused_kernel, dim3(groups), dim3(kBlock), 0, stream,
-                     w.st, w.partials, grid, w.groups, tw.gw, tw.fw, do_solve,
+    launch_jacobian_nb<S, GEOM, FEAT, 4>(L, js, w, rec, tw, fixed_map, stream);
+  if (do_solve < 0) return;
+  hipLaunchKernelGGL(solve_fused_kernel, dim3(1), dim3(kSolveBlock), 0, stream,
+                     w.st, w.partials, js.grid, tw.gw, tw.fw, do_solve,
                      fixed_map);
+}
+
+template <typename S>
+void launch_jacobian_terms(const stb_icp_level& L, const stb_icp_params& P,
+                           const Workspace& w, const SolveRecord* rec,
+                           int do_solve, int fixed_map, hipStream_t stream) {
+  const TermWeights tw = make_weights(P);
+  if (P.use_geom && P.use_feat)
+    launch_jacobian<S, true, true>(L, P, w, rec, tw, do_solve, fixed_map, stream);
+  else if (P.use_geom)
+    launch_jacobian<S, true, false>(L, P, w, rec, tw, do_solve, fixed_map, stream);
+  else
+    launch_jacobian<S, false, true>(L, P, w, rec, tw, do_solve, fixed_map, stream);
 }
 
 template <typename S>
@@ -309,22 +453,33 @@ void launch_phase(const stb_icp_level& L, const stb_icp_params& P,
                        w.partials);
     return;
   }
-  const TermWeights tw = make_weights(P);
   const int solve = phase == kUpdate ? 1 : (phase == kJacobians ? -1 : 0);
-  if (P.use_geom && P.use_feat)
-    launch_jacobian<S, true, true>(L, P, w, tw, solve, fixed, stream);
-  else if (P.use_geom)
-    launch_jacobian<S, true, false>(L, P, w, tw, solve, fixed, stream);
-  else
-    launch_jacobian<S, false, true>(L, P, w, tw, solve, fixed, stream);
+  launch_jacobian_terms<S>(L, P, w, nullptr, solve, fixed, stream);
 }
 
+// `count` iterations in 2 * count + 1 launches: the first iteration's
+// correspondence and Jacobian launches read the state; each later
+// correspondence launch applies the previous iteration's update (SolveRecord)
+// and its Jacobian launch commits it; one solve launch applies the last.
 template <typename S>
 void launch_iterations(const stb_icp_level& L, const stb_icp_params& P,
                        const Workspace& w, int count, hipStream_t stream) {
+  const float dthr2 = float(P.distance_thresh * P.distance_thresh);
+  const float cstar = angle_gate_cos_f32(P.normal_angle_thresh);
+  const int partials = jac_shape(P, L.n_src).grid;
   for (int it = 0; it < count; ++it) {
-    launch_phase<S>(L, P, w, kCorrespond, stream);
-    launch_phase<S>(L, P, w, kUpdate, stream);
+    if (it == 0)
+      hipLaunchKernelGGL(corresp_fused_kernel<S>,
+                         dim3(corresp_grid(P, L.n_src)), dim3(kBlock), 0,
+                         stream, L, w.st, w.map[0], w.map[1], dthr2, cstar, 0);
+    else
+      hipLaunchKernelGGL(corresp_solve_kernel<S>,
+                         dim3(corresp_solve_grid(P, L.n_src)),
+                         dim3(kSolveBlock), 0, stream, L, w.st, w.rec,
+                         w.partials, partials, w.map[0], w.map[1], dthr2,
+                         cstar);
+    launch_jacobian_terms<S>(L, P, w, it ? w.rec : nullptr,
+                             it + 1 == count ? 1 : -1, 0, stream);
   }
 }
 

@@ -95,15 +95,16 @@ __device__ inline LevelView<S> view_of(const stb_icp_level& L) {
 // loads it depends on and reads their results, so the compiler cannot sink a
 // load below an earlier test: the source loads issue together, then the
 // target loads, and a pixel costs two memory round trips.
+// corresp_pixel from the pixel's already loaded source mask, point and
+// normal.
 template <typename S>
-__device__ inline void corresp_pixel(const LevelView<S>& L, const float* m,
-                                     const float* nm, int64_t idx,
-                                     unsigned long long* __restrict__ map,
-                                     float dthr2, float cstar) {
+__device__ inline void corresp_pixel_at(const LevelView<S>& L, const float* m,
+                                        const float* nm, int64_t idx,
+                                        uint8_t valid_src, V3<float> ps,
+                                        V3<float> ns,
+                                        unsigned long long* __restrict__ map,
+                                        float dthr2, float cstar) {
 #pragma clang fp contract(off)
-  const uint8_t valid_src = L.sm[idx];
-  const V3<float> ps = ld3f(L.sp, idx);
-  const V3<float> ns = ld3f(L.sn, idx);
   const V3<float> p = rigid_point(m, ps);
   const V3<float> nn = rigid_normal(nm, ns);
   float u, v;
@@ -122,6 +123,17 @@ __device__ inline void corresp_pixel(const LevelView<S>& L, const float* m,
                     !angle_rejects(gate_cos(nn, qn), cstar);
   if (!keep) return;
   atomicMin(map + pos, (unsigned long long)make_key(sqrtf(sq), uint32_t(idx)));
+}
+
+template <typename S>
+__device__ inline void corresp_pixel(const LevelView<S>& L, const float* m,
+                                     const float* nm, int64_t idx,
+                                     unsigned long long* __restrict__ map,
+                                     float dthr2, float cstar) {
+  const uint8_t valid_src = L.sm[idx];
+  const V3<float> ps = ld3f(L.sp, idx);
+  const V3<float> ns = ld3f(L.sn, idx);
+  corresp_pixel_at(L, m, nm, idx, valid_src, ps, ns, map, dthr2, cstar);
 }
 
 // Adds one weighted term J' = w J, r' (already scaled) to the JtJ/Jr slots.
@@ -150,11 +162,19 @@ __device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
 // loads issue in one batch: a pixel costs two memory round trips. A pixel
 // that does not own its target adds exact zeros (selected, never multiplied
 // in, so NaN or inf in its unused data cannot leak into the sums).
+// With `clear` (a Gauss-Newton loop's own map) the owner of a map entry
+// resets it to the sentinel after reading it: every entry the
+// correspondence launch set has an owner that projects onto it again here,
+// so the map is empty again when the launch ends, without a sweep over it.
+// Other pixels projecting onto the entry read either the owner's index or
+// the sentinel, both "not mine". The store comes last, after every load of
+// the pixel has issued: the map is not a restrict pointer (it is written),
+// so no later load could move above the store.
 template <typename S, bool GEOM, bool FEAT, int NB = 4>
 __device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
-                                      const unsigned long long* __restrict__ map,
+                                      const unsigned long long* map,
                                       int64_t idx, const TermWeights& tw,
-                                      float (&acc)[kSlots]) {
+                                      float (&acc)[kSlots], bool clear = false) {
 #pragma clang fp contract(off)
   const uint8_t valid_src = L.sm[idx];
   const V3<float> ps = ld3f(L.sp, idx);
@@ -195,6 +215,7 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
     acc[28] += use ? sq : 0.0f;
     acc[30] += use ? 1.0f : 0.0f;
   }
+  if (clear && own) const_cast<unsigned long long*>(map)[pos] = kKeySentinel;
 }
 
 // G^T G of source pixel idx from its matched target point
@@ -263,7 +284,8 @@ __device__ inline int bitrev5(int l) {
 // With `publish` the stores are fenced device-wide, for a consumer in the
 // same launch (arrive_last); a consumer in a later launch needs no fence, and
 // each fence costs a few nanoseconds per block at the memory side.
-// `sh` is LDS of kShared doubles.
+// `sh` is LDS of BLOCK / 64 * kSlots doubles (kShared for kBlock).
+template <int BLOCK = kBlock>
 __device__ inline void block_partial(float (&acc)[kSlots], double* out,
                                      double* sh, bool publish) {
   const int lane = threadIdx.x & 63;
@@ -274,7 +296,7 @@ __device__ inline void block_partial(float (&acc)[kSlots], double* out,
   if (threadIdx.x < kSlots) {
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) s += sh[w * kSlots + threadIdx.x];
+    for (int w = 0; w < BLOCK / 64; ++w) s += sh[w * kSlots + threadIdx.x];
     out[threadIdx.x] = s;
     if (publish) __threadfence();
   }
@@ -293,18 +315,22 @@ __device__ inline bool arrive_last(int32_t* counter, int expected,
 }
 
 // Fixed-order FP64 sum of `count` partials (32 doubles each) into tot[32] by
-// a block of BLOCK threads: group g of BLOCK / 32 adds partials g,
-// g + BLOCK / 32, ..., then the groups are added in order -- the same order
-// whichever block performs it. The partials were written by blocks on every
-// XCD, so each load misses this XCD's L2: kSumBatch independent loads are
-// issued before their in-order adds, paying that latency once per batch
-// instead of once per partial. `sh` is LDS of BLOCK doubles.
-constexpr int kSumBatch = 16;
+// a block of BLOCK threads, the same order whichever block performs it:
+// thread group g of BLOCK / 32 takes partials g, g + BLOCK / 32, ... in
+// batches of kSumBatch summed pairwise; the two groups of a wavefront are
+// added across its halves, and the per-wavefront sums pairwise through LDS.
+// Pairwise sums keep the dependent chain short (log2 instead of linear in the
+// number of terms): this sum sits on the critical path of every iteration.
+// The partials were written by blocks on every XCD, so each load misses
+// this XCD's L2: a batch's loads issue before any of its adds. `sh` is LDS of
+// BLOCK / 64 * kSlots doubles.
+constexpr int kSumBatch = 8;
 
 template <int BLOCK>
 __device__ inline void sum_partials(const double* partials, int count,
                                     double* sh, double* tot) {
   constexpr int kG = BLOCK / kSlots;
+  constexpr int kW = BLOCK / 64;
   const int slot = threadIdx.x & (kSlots - 1);
   const int grp = threadIdx.x / kSlots;
   double s = 0.0;
@@ -312,20 +338,31 @@ __device__ inline void sum_partials(const double* partials, int count,
     double v[kSumBatch];
 #pragma unroll
     for (int u = 0; u < kSumBatch; ++u) {
+      // Loads past the end re-read the last partial and are dropped, so
+      // the batch issues without branches.
       const int b = b0 + u * kG;
-      v[u] = b < count ? partials[int64_t(b) * kSlots + slot] : 0.0;
+      const double x = partials[int64_t(b < count ? b : count - 1) * kSlots + slot];
+      v[u] = b < count ? x : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < kSumBatch; ++u) s += v[u];
+    for (int w = kSumBatch / 2; w > 0; w /= 2)
+#pragma unroll
+      for (int u = 0; u < w; ++u) v[u] = v[u] + v[u + w];
+    s += v[0];
   }
+  s = s + __shfl_xor(s, 32, 64);  // groups 2w and 2w + 1 of wavefront w
   __syncthreads();
-  sh[grp * kSlots + slot] = s;
+  if ((threadIdx.x & 63) < kSlots) sh[(threadIdx.x >> 6) * kSlots + slot] = s;
   __syncthreads();
   if (threadIdx.x < kSlots) {
-    double t = 0.0;
+    double g[kW];
 #pragma unroll
-    for (int g = 0; g < kG; ++g) t += sh[g * kSlots + threadIdx.x];
-    tot[threadIdx.x] = t;
+    for (int w = 0; w < kW; ++w) g[w] = sh[w * kSlots + threadIdx.x];
+#pragma unroll
+    for (int h = kW / 2; h > 0; h /= 2)
+#pragma unroll
+      for (int w = 0; w < h; ++w) g[w] = g[w] + g[w + h];
+    tot[threadIdx.x] = g[0];
   }
   __syncthreads();
 }
@@ -349,30 +386,73 @@ __device__ inline void set_pose_floats(stb_icp_state* st) {
 
 // exp_rt_to_matrix in FP64 (src/exp_rt.hpp:27-65, src/exp_rt_to_matrix.cu:22-36):
 // Rodrigues rotation of w = x[3:6], translation x[0:3] used as is.
+// For |w| <= 1/2 (every Gauss-Newton step of a converging loop) the same
+// rotation is formed as R = I + A [w]x + B [w]x^2, with A = sin t / t and
+// B = (1 - cos t) / t^2 summed from their Taylor series in t^2 = |w|^2 to the
+// term in t^16 (truncation below 1e-22): no square root, division or
+// trigonometric range reduction on the solve's serial chain. Larger angles
+// take the reference's axis-angle form.
 __device__ inline void exp_rt_f64(const double x[6], double U[12]) {
   const double wx = x[3], wy = x[4], wz = x[5];
-  const double sq = wx * wx + (wy * wy + wz * wz);
-  const double th = sqrt(sq);
-  double ax = wx, ay = wy, az = wz;
-  if (sq > 0.0) {
-    ax = wx / th;
-    ay = wy / th;
-    az = wz / th;
+  const double xx = wx * wx, yy = wy * wy, zz = wz * wz;
+  const double sq = xx + (yy + zz);
+  double r00, r01, r02, r10, r11, r12, r20, r21, r22;
+  if (sq <= 0.25) {
+    double p = -1.0 / 355687428096000.0;  // (1 - A) / t^2, highest term first
+    p = fma(p, sq, 1.0 / 1307674368000.0);
+    p = fma(p, sq, -1.0 / 6227020800.0);
+    p = fma(p, sq, 1.0 / 39916800.0);
+    p = fma(p, sq, -1.0 / 362880.0);
+    p = fma(p, sq, 1.0 / 5040.0);
+    p = fma(p, sq, -1.0 / 120.0);
+    p = fma(p, sq, 1.0 / 6.0);
+    const double A = fma(-p, sq, 1.0);
+    double B = 1.0 / 6402373705728000.0;
+    B = fma(B, sq, -1.0 / 20922789888000.0);
+    B = fma(B, sq, 1.0 / 87178291200.0);
+    B = fma(B, sq, -1.0 / 479001600.0);
+    B = fma(B, sq, 1.0 / 3628800.0);
+    B = fma(B, sq, -1.0 / 40320.0);
+    B = fma(B, sq, 1.0 / 720.0);
+    B = fma(B, sq, -1.0 / 24.0);
+    B = fma(B, sq, 0.5);
+    const double bxy = B * wx * wy, bxz = B * wx * wz, byz = B * wy * wz;
+    r00 = 1.0 - B * (yy + zz);
+    r11 = 1.0 - B * (xx + zz);
+    r22 = 1.0 - B * (xx + yy);
+    r01 = bxy - A * wz;
+    r10 = bxy + A * wz;
+    r02 = bxz + A * wy;
+    r20 = bxz - A * wy;
+    r12 = byz - A * wx;
+    r21 = byz + A * wx;
+  } else {
+    const double th = sqrt(sq);
+    const double ax = wx / th, ay = wy / th, az = wz / th;
+    double s, c;
+    sincos(th, &s, &c);
+    const double k = 1.0 - c;
+    r00 = k * ax * ax + c;
+    r01 = k * ax * ay - s * az;
+    r02 = k * ax * az + s * ay;
+    r10 = k * ax * ay + s * az;
+    r11 = k * ay * ay + c;
+    r12 = k * ay * az - s * ax;
+    r20 = k * ax * az - s * ay;
+    r21 = k * ay * az + s * ax;
+    r22 = k * az * az + c;
   }
-  double s, c;
-  sincos(th, &s, &c);
-  const double k = 1.0 - c;
-  U[0] = k * ax * ax + c;
-  U[1] = k * ax * ay - s * az;
-  U[2] = k * ax * az + s * ay;
-  U[4] = k * ax * ay + s * az;
-  U[5] = k * ay * ay + c;
-  U[6] = k * ay * az - s * ax;
-  U[8] = k * ax * az - s * ay;
-  U[9] = k * ay * az + s * ax;
-  U[10] = k * az * az + c;
+  U[0] = r00;
+  U[1] = r01;
+  U[2] = r02;
   U[3] = x[0];
+  U[4] = r10;
+  U[5] = r11;
+  U[6] = r12;
   U[7] = x[1];
+  U[8] = r20;
+  U[9] = r21;
+  U[10] = r22;
   U[11] = x[2];
 }
 
@@ -471,10 +551,10 @@ __device__ inline void gn_finalize(stb_icp_state* st, const double* tot,
         y[i] = v * rinv[i];
       }
 #pragma unroll
-      for (int i = 5; i >= 0; --i) {  // L^T x = y
+      for (int i = 5; i >= 0; --i) {  // L^T x = y, the newest x[q] last
         double v = y[i];
 #pragma unroll
-        for (int q = i + 1; q < 6; ++q) v -= L[tri_lo(q, i)] * x[q];
+        for (int q = 5; q > i; --q) v -= L[tri_lo(q, i)] * x[q];
         x[i] = v * rinv[i];
       }
       double U[12];
@@ -543,6 +623,167 @@ __device__ inline void finish_partials(stb_icp_state* st, const double* partials
     gn_finalize(st, tot, gw, fw, sh, iters, sel);
   } else {
     store_linearization(st, tot);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pose update inside the next iteration's correspondence launch
+// ---------------------------------------------------------------------------
+//
+// Iteration k's correspondence launch can apply iteration k-1's update
+// itself: each of its blocks sums the previous Jacobian launch's partials in
+// the same fixed order, solves the 6x6 system and forms the new pose, so no
+// launch (and no cross-block ticket) stands between a Jacobian launch and the
+// next correspondence pass. Those blocks still read the state, so the update
+// goes to a SolveRecord written by block 0; the Jacobian launch of the same
+// iteration takes its pose and map from the record, and its block 0 commits
+// the record to the state.
+struct SolveRecord {
+  double tot[kSlots];  // reduced system of the previous iteration
+  double T[12];        // rows 0..2 of the updated pose (when ok)
+  float rt[12];        // float pose this iteration uses
+  float nm[9];         // its normal matrix
+  int32_t active;      // 0: the loop had already stopped, nothing to commit
+  int32_t ok;          // every Cholesky pivot was positive
+  int32_t map_sel;     // correspondence map this iteration uses
+  int32_t iterations;  // state.iterations after a successful update
+};
+
+__device__ inline double readlane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(b), lane));
+  const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(b >> 32), lane));
+  return __longlong_as_double(
+      (long long)((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Cholesky solve of the system in `tot` and the pose update, spread over the
+// 64 lanes of one wavefront (every lane calls it). Lane 6i+k (k < 6) holds
+// element (i, k) of the trailing matrix and lane 36+i element i of the
+// right-hand side; eliminating column j, every lane (i, k) with i, k > j
+// subtracts L[i][j] L[k][j], and lane 36+i (i > j) subtracts L[i][j] y[j].
+// Those are the subtractions of gn_finalize's one-thread left-looking
+// factorisation and forward substitution, in the same order and on the same
+// operands, so both give the same bits -- and the forward substitution
+// rides along with the factorisation instead of following it. The back
+// substitution and exp_rt run on wave-uniform values as in gn_finalize.
+// `To` (LDS) holds rows 0..2 of the current pose; lanes 0..11 leave the
+// updated rows in `Tn` and their float values in `rtf`. Returns whether every
+// pivot was positive.
+__device__ inline bool wave_solve_update(const double* tot, const double* To,
+                                         double* Tn, float* rtf) {
+  const int l = threadIdx.x & 63;
+  const bool elem = l < 36;
+  const bool rhs = l >= 36 && l < 42;
+  const int i = elem ? l / 6 : (rhs ? l - 36 : 0);
+  const int k = elem ? l % 6 : 0;
+  double a = elem ? sym6(tot, i, k) : (rhs ? tot[21 + i] : 0.0);
+  // Partner of lane (i, k) in the update: L[k][j] (lane 6k + j) for the
+  // matrix, y[j] (lane 36 + j) for the right-hand side.
+  const int pbase = rhs ? 36 : 6 * k;
+  double rinv[6];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const double d2 = readlane_f64(a, 7 * j);
+    ok = ok && (d2 > 0.0);
+    double r = __builtin_amdgcn_rsq(d2);
+    r = fma(r * 0.5, fma(-d2 * r, r, 1.0), r);
+    r = fma(r * 0.5, fma(-d2 * r, r, 1.0), r);
+    rinv[j] = r;
+    const double lij = __shfl(a, 6 * i + j, 64) * r;  // L[i][j]
+    const double lkj = __shfl(a, pbase + j, 64) * r;  // L[k][j] or y[j]
+    if ((elem || rhs) && i > j && (rhs || k > j)) a = a - lij * lkj;
+    if (elem && k == j && i > j) a = lij;  // this lane holds L[i][j] now
+  }
+  double y[6], x[6];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) y[r] = readlane_f64(a, 36 + r) * rinv[r];
+#pragma unroll
+  for (int r = 5; r >= 0; --r) {  // L^T x = y, the newest x[q] last
+    double v = y[r];
+#pragma unroll
+    for (int q = 5; q > r; --q) v -= readlane_f64(a, 6 * q + r) * x[q];
+    x[r] = v * rinv[r];
+  }
+  double U[12];
+  exp_rt_f64(x, U);
+  if (l < 12) {
+    const int row = l >> 2, c = l & 3;
+    const double u0 = row == 0 ? U[0] : (row == 1 ? U[4] : U[8]);
+    const double u1 = row == 0 ? U[1] : (row == 1 ? U[5] : U[9]);
+    const double u2 = row == 0 ? U[2] : (row == 1 ? U[6] : U[10]);
+    const double u3 = row == 0 ? U[3] : (row == 1 ? U[7] : U[11]);
+    double v = u0 * To[c] + u1 * To[4 + c] + u2 * To[8 + c];
+    if (c == 3) v += u3;
+    Tn[l] = v;
+    rtf[l] = float(v);
+  }
+  return ok;
+}
+
+// What gn_finalize stores, from a SolveRecord (every thread of a block of at
+// least 102 threads calls it). Each thread loads the values it stores first,
+// so the block pays one memory round trip, not one per field.
+__device__ inline void commit_record(stb_icp_state* __restrict__ st,
+                                     const SolveRecord* __restrict__ rec,
+                                     double gw, double fw) {
+  const int t = threadIdx.x;
+  const int active = rec->active, ok = rec->ok;
+  const double* __restrict__ tot = rec->tot;
+  double v0 = 0.0, v1 = 0.0;
+  float f0 = 0.0f;
+  int i0 = 0, i1 = 0;
+  if (t < 36)
+    v0 = sym6(tot, t / 6, t % 6);  // last_jtj
+  else if (t < 46)
+    v0 = tot[21 + t - 36];  // last_jr, r2 and counts (slots 21..30)
+  if (t < 12) {
+    v1 = rec->T[t];
+    f0 = rec->rt[t];
+  } else if (t < 21) {
+    f0 = rec->nm[t - 12];
+  } else if (t >= 64 && t < 100) {
+    v1 = sym6(tot, (t - 64) / 6, (t - 64) % 6);
+  } else if (t == 100) {  // best_residual and best_match_count
+    const double c = fmax(tot[29], tot[30]);
+    v0 = (fw * tot[28] + gw * tot[27]) / c;
+    v1 = c;
+  } else if (t == 101) {
+    i0 = rec->iterations;
+    i1 = rec->map_sel;
+  }
+  if (!active) return;
+  if (t < 36)
+    st->last_jtj[t] = v0;
+  else if (t < 42)
+    st->last_jr[t - 36] = v0;
+  else if (t == 42)
+    st->last_geom_r2 = v0;
+  else if (t == 43)
+    st->last_feat_r2 = v0;
+  else if (t == 44)
+    st->last_geom_count = v0;
+  else if (t == 45)
+    st->last_feat_count = v0;
+  if (!ok) {
+    if (t == 0) st->stopped = 1;
+    return;
+  }
+  if (t < 12) {
+    st->transform[t] = v1;
+    st->best_transform[t] = v1;
+    st->rt[t] = f0;
+  } else if (t < 21) {
+    st->nm[t - 12] = f0;
+  } else if (t >= 64 && t < 100) {
+    st->best_jtj[t - 64] = v1;
+  } else if (t == 100) {
+    st->best_residual = v0;
+    st->best_match_count = v1;
+  } else if (t == 101) {
+    st->iterations = i0;
+    st->map_sel = i1;
   }
 }
 

@@ -149,7 +149,8 @@ class GNRunner:
                 _lib.stream()), "stb_icp_linearize")
 
     def jacobians(self, level, params):
-        """The Jacobian launch alone over the last iteration's map (timing)."""
+        """The Jacobian launch alone over the last ``linearize`` call's map
+        (timing)."""
         ws, nbytes = self._ws()
         with torch.cuda.device(self.device):
             _lib.check(_lib.lib().stb_icp_jacobians(

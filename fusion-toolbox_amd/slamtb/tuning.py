@@ -12,8 +12,11 @@ import torch
 
 from slamtb.registration._fused import runner_for
 
-UPDATE_CANDIDATES = (0, 512, 2048, 4096)  # 0: the library default (1024)
-CORRESP_CANDIDATES = (0, 1024, 2048)  # 0: one pixel per thread up to 4096 blocks
+# 0: the library default shape (at most 256 Jacobian blocks, of 1024 threads
+# above 131,072 pixels); a count: that many blocks of 256 threads.
+UPDATE_CANDIDATES = (0, 512, 1024, 2048)
+# 0: one pixel per thread up to the library's caps; a count: that cap.
+CORRESP_CANDIDATES = (0, 256, 1024, 2048)
 TILE_CANDIDATES = (256, 512, 1024, 2048)
 
 _TABLE = {}
@@ -48,14 +51,12 @@ def tune_level(level, params, reps=20):
     run.begin(level)
     run.iterate(level, params, 2)
     best_u, best_t = 0, float("inf")
+    # Whole loops are timed: a loop's correspondence launches also apply
+    # the previous iteration's update, so their cost depends on both shapes.
     for g in UPDATE_CANDIDATES:
         params.grid_update = g
-
-        def step():
-            run.correspond(level, params)
-            run.update(level, params)
         run.begin(level)
-        t = _time(step, reps)
+        t = _time(lambda: run.iterate(level, params, 5), reps)
         if t < best_t:
             best_u, best_t = g, t
     params.grid_update = best_u
@@ -63,7 +64,7 @@ def tune_level(level, params, reps=20):
     for g in CORRESP_CANDIDATES:
         params.grid_corresp = g
         run.begin(level)
-        t = _time(lambda: run.correspond(level, params), reps)
+        t = _time(lambda: run.iterate(level, params, 5), reps)
         if t < best_t:
             best_c, best_t = g, t
     params.grid_corresp = best_c

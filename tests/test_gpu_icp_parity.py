@@ -255,3 +255,70 @@ def test_icp_zero_overlap_keeps_no_transform(pair320):
     res = run_icp(pair320, 5, T0=far)
     assert res.transform is None and res.iterations == 0
     assert math.isinf(res.residual)
+
+
+def state_bits(st):
+    return dict(transform=tuple(st.transform), best_jtj=tuple(st.best_jtj),
+                last_jtj=tuple(st.last_jtj), last_jr=tuple(st.last_jr),
+                residual=st.best_residual, count=st.best_match_count,
+                r2=(st.last_geom_r2, st.last_feat_r2), rt=tuple(st.rt),
+                nm=tuple(st.nm), iterations=st.iterations, stopped=st.stopped,
+                map_sel=st.map_sel)
+
+
+def midstop_pair():
+    """A 160x120 pair and start pose from which the oracle's loop takes one
+    step and then stops on a non-positive pivot (few matches left)."""
+    k = synthetic.intrinsics(160, 120).astype(np.float32)
+    P = prepared_pair(3, 160, 120, kcam=k)
+    rng = np.random.default_rng(4)
+    synthetic.random_se3(rng, 20, 0.3)
+    return P, synthetic.random_se3(rng, 30, 0.5)
+
+
+@pytest.mark.parametrize("case", ["geom320", "both640", "midstop"])
+def test_loop_equals_split_iterations(case):
+    """stb_icp_iterate applies each update inside the next iteration's
+    correspondence launch (every block sums the partials and solves; the
+    factorisation spread over a wavefront); stb_icp_correspond +
+    stb_icp_update apply it in a one-block launch with the one-thread
+    factorisation. Same sums, same subtractions in the same order: the two
+    leave bit-identical states, also when the loop stops part-way."""
+    gw, fw, dthr, iters = 1.0, 0.0, 0.1, 10
+    if case == "geom320":
+        P, T0 = prepared_pair(9, 320, 240), small_motion()
+    elif case == "both640":  # 1024-thread Jacobian blocks
+        P, T0 = prepared_pair(7, 640, 480, channels=3), small_motion(5)
+        gw, fw, iters = 10.0, 1.0, 5
+    else:
+        P, T0 = midstop_pair()
+        dthr = 0.05
+    g = to_dev(P, DEV)
+    level = Level(P["kcam"], g["src_pts"], g["src_nrm"], g["src_mask"],
+                  g["tgt_pts"], g["tgt_nrm"], g["tgt_mask"],
+                  g["src_feats"] if fw else None, g["tgt_feats"] if fw else None)
+    params = make_params(gw, fw, dthr, ANG20, 2.75, fw > 0)
+    run = runner_for(DEV)
+    run.begin(level, torch.from_numpy(T0))
+    run.iterate(level, params, iters)
+    loop = state_bits(run.read_state())
+    run.begin(level, torch.from_numpy(T0))
+    for _ in range(iters):
+        run.correspond(level, params)
+        run.update(level, params)
+    split = state_bits(run.read_state())
+    print(case, "iterations", loop["iterations"], "stopped", loop["stopped"])
+    for key in loop:
+        assert loop[key] == split[key], key
+    if case == "midstop":
+        ref = orc.icp_estimate(P["kcam"], P["src_pts"], P["src_nrm"], P["src_mask"],
+                               P["tgt_pts"], P["tgt_nrm"], P["tgt_mask"], iters,
+                               geom_weight=gw, feat_weight=fw,
+                               distance_threshold=dthr, normals_angle_thresh=ANG20,
+                               transform=T0)
+        assert ref["iters"] == 1 and loop["iterations"] == 1 and loop["stopped"] == 1
+        T = np.array(loop["transform"]).reshape(4, 4)
+        err = orc.pose_error(T, ref["transform"])
+        assert err[0] < 1e-6 and err[1] < 1e-6, err
+    else:
+        assert loop["iterations"] == iters and loop["stopped"] == 0
