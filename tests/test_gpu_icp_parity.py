@@ -322,3 +322,42 @@ def test_loop_equals_split_iterations(case):
         assert err[0] < 1e-6 and err[1] < 1e-6, err
     else:
         assert loop["iterations"] == iters and loop["stopped"] == 0
+
+
+def workspace_maps(run, level):
+    """The two correspondence maps inside the fused workspace (state, then
+    each map padded to 256 bytes), as uint64 tensors."""
+    from slamtb.registration._fused import STATE_BYTES
+    pad = lambda b: (b + 255) // 256 * 256
+    hw = level.height * level.width
+    off0 = pad(STATE_BYTES)
+    off1 = off0 + pad(hw * 8)
+    ws = run.workspace
+    return [ws[o:o + hw * 8].view(torch.int64) for o in (off0, off1)]
+
+
+def test_loop_leaves_both_maps_empty():
+    """A loop's Jacobian launches empty the correspondence map as its owners
+    read it (no sweep), and the loop's first iteration sweeps the map a
+    one-off pass left set: after any loop both maps hold only the sentinel,
+    so the next correspondence pass starts from an empty map."""
+    sentinel = 0x7f8000000fffffff
+    P = prepared_pair(7, 640, 480, channels=3)
+    g = to_dev(P, DEV)
+    level = Level(P["kcam"], g["src_pts"], g["src_nrm"], g["src_mask"],
+                  g["tgt_pts"], g["tgt_nrm"], g["tgt_mask"], g["src_feats"],
+                  g["tgt_feats"])
+    params = make_params(10.0, 1.0, 0.1, ANG20, 2.75, True)
+    run = runner_for(DEV)
+    run.begin(level, torch.from_numpy(small_motion(5)))
+    run.iterate(level, params, 4)
+    for m in workspace_maps(run, level):
+        assert bool((m == sentinel).all())
+    run.linearize(level, params)  # sets the spare map
+    st = run.read_state()
+    assert max(st.last_geom_count, st.last_feat_count) > 10000
+    assert any(not bool((m == sentinel).all()) for m in workspace_maps(run, level))
+    run.iterate(level, params, 3)
+    for m in workspace_maps(run, level):
+        assert bool((m == sentinel).all())
+    assert run.read_state().iterations == 7
