@@ -152,7 +152,7 @@ __device__ inline void scatter5(float* __restrict__ ch, int W, const BilD& c0,
 
 // Block sum of the 12 pose-gradient terms in FP64 (lanes by shuffles, waves
 // through LDS), written as one kSlots-wide partial (unused slots zero) for
-// the fixed-order sum of the last block.
+// the fixed-order sum of adjoint_sum_kernel (a later launch: no fence).
 __device__ inline void pose_partial(double (&acc)[kPose], double* out,
                                     double* sh) {
   const int lane = threadIdx.x & 63;
@@ -171,7 +171,6 @@ __device__ inline void pose_partial(double (&acc)[kPose], double* out,
 #pragma unroll
       for (int w = 0; w < kWaves; ++w) s += sh[w * kPose + threadIdx.x];
     out[threadIdx.x] = s;
-    __threadfence();
   }
   __syncthreads();
 }
@@ -325,12 +324,8 @@ __global__ void __launch_bounds__(kBlock)
                          unsigned long long* __restrict__ maps1,
                          const stb_icp_grads* __restrict__ grads,
                          const double* __restrict__ coef,
-                         double* __restrict__ dT,
-                         double* __restrict__ partials,
-                         int32_t* __restrict__ tickets, TermWeights tw) {
+                         double* __restrict__ partials, TermWeights tw) {
   __shared__ double sh[kShared];
-  __shared__ double tot[kSlots];
-  __shared__ int s_last;
   const PlanView pv = plan_view(plan);
   for (int t = blockIdx.x; t < pv.tiles; t += gridDim.x) {
     const int p = pv.tile_pair[t];
@@ -346,9 +341,7 @@ __global__ void __launch_bounds__(kBlock)
     }
 #pragma unroll
     for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
-    const int first = pv.tile_start[p];
-    const int ntiles = pv.tile_start[p + 1] - first;
-    const int64_t base = int64_t(t - first) * pv.tile_px;
+    const int64_t base = int64_t(t - pv.tile_start[p]) * pv.tile_px;
     double tacc[kPose];
 #pragma unroll
     for (int k = 0; k < kPose; ++k) tacc[k] = 0.0;
@@ -358,22 +351,36 @@ __global__ void __launch_bounds__(kBlock)
         adjoint_pixel<GEOM, FEAT>(L, m, map, idx, tw, lam, x, G, tacc);
     }
     pose_partial(tacc, partials + int64_t(t) * kSlots, sh);
-    if (arrive_last(&tickets[p], ntiles - 1, &s_last)) {
-      sum_partials<kBlock>(partials + int64_t(first) * kSlots, ntiles, sh, tot);
-      if (threadIdx.x < 12) dT[12 * p + threadIdx.x] = tot[threadIdx.x];
-      if (threadIdx.x == 0) tickets[p] = 0;
-    }
-    __syncthreads();
   }
 }
 
+// One block per pair: the fixed-order sum of the pair's tile partials into
+// its 12 dL/dT terms.
+__global__ void __launch_bounds__(kSolveBlock)
+    adjoint_sum_kernel(const void* __restrict__ plan,
+                       const double* __restrict__ partials,
+                       double* __restrict__ dT) {
+  __shared__ double sh[kSolveBlock];
+  __shared__ double tot[kSlots];
+  const PlanView pv = plan_view(plan);
+  const int p = blockIdx.x;
+  if (p >= pv.P) return;
+  const int first = pv.tile_start[p];
+  sum_partials<kSolveBlock>(partials + int64_t(first) * kSlots,
+                            pv.tile_start[p + 1] - first, sh, tot);
+  if (threadIdx.x < 12) dT[12 * p + threadIdx.x] = tot[threadIdx.x];
+}
+
 template <bool GEOM, bool FEAT>
-void launch_adjoint(const void* plan, stb_icp_state* st, const Scratch& sc,
-                    int grid, const stb_icp_grads* grads, const double* coef,
-                    double* dT, const TermWeights& tw, hipStream_t s) {
+void launch_adjoint(const void* plan, int num_pairs, stb_icp_state* st,
+                    const Scratch& sc, int grid, const stb_icp_grads* grads,
+                    const double* coef, double* dT, const TermWeights& tw,
+                    hipStream_t s) {
   hipLaunchKernelGGL((batch_adjoint_kernel<GEOM, FEAT>), dim3(grid),
                      dim3(kBlock), 0, s, plan, st, sc.map0, sc.map1, grads,
-                     coef, dT, sc.partials, sc.tickets, tw);
+                     coef, sc.partials, tw);
+  hipLaunchKernelGGL(adjoint_sum_kernel, dim3(num_pairs), dim3(kSolveBlock), 0,
+                     s, plan, sc.partials, dT);
 }
 
 }  // namespace
@@ -411,11 +418,12 @@ extern "C" int stb_icp_batch_adjoint(const void* plan,
   const TermWeights tw = make_weights(*params);
   const int cap = params->grid_update > 0 ? params->grid_update : 1024;
   const int grid = sizes->total_tiles < cap ? sizes->total_tiles : cap;
+  const int P = sizes->num_pairs;
   if (params->use_geom && params->use_feat)
-    launch_adjoint<true, true>(plan, st, sc, grid, grads, coef, dT, tw, s);
+    launch_adjoint<true, true>(plan, P, st, sc, grid, grads, coef, dT, tw, s);
   else if (params->use_geom)
-    launch_adjoint<true, false>(plan, st, sc, grid, grads, coef, dT, tw, s);
+    launch_adjoint<true, false>(plan, P, st, sc, grid, grads, coef, dT, tw, s);
   else
-    launch_adjoint<false, true>(plan, st, sc, grid, grads, coef, dT, tw, s);
+    launch_adjoint<false, true>(plan, P, st, sc, grid, grads, coef, dT, tw, s);
   return check_launch("stb_icp_batch_adjoint");
 }

@@ -87,17 +87,18 @@ __global__ void __launch_bounds__(kBlock)
       const int64_t idx = base + k;
       if (idx < L.n) jacobian_pixel<S, GEOM, FEAT, NB>(L, m, map, idx, tw, acc);
     }
-    block_partial(acc, partials + int64_t(t) * kSlots, sh, false);
+    block_partial(acc, partials + int64_t(t) * kSlots, sh);
   }
 }
 
 // One block per pair: the fixed-order sum of the pair's tile partials, then
-// its pose update (do_solve) or stored linearisation.
+// its pose update, stored linearisation or information matrix (`mode`, a
+// FinishMode).
 __global__ void __launch_bounds__(kSolveBlock)
     batch_solve_kernel(const void* __restrict__ plan,
                        stb_icp_state* __restrict__ states,
                        const double* __restrict__ partials, double gw,
-                       double fw, int do_solve, int fixed_map) {
+                       double fw, int mode, int fixed_map) {
   __shared__ double sh[kSolveBlock];
   __shared__ double tot[kSlots];
   const PlanView pv = plan_view(plan);
@@ -107,7 +108,7 @@ __global__ void __launch_bounds__(kSolveBlock)
   if (!fixed_map && st->stopped) return;
   const int first = pv.tile_start[p];
   finish_partials<kSolveBlock>(st, partials + int64_t(first) * kSlots,
-                  pv.tile_start[p + 1] - first, do_solve, gw, fw, sh, tot);
+                  pv.tile_start[p + 1] - first, mode, gw, fw, sh, tot);
 }
 
 template <typename S>
@@ -116,11 +117,8 @@ __global__ void __launch_bounds__(kBlock)
                              stb_icp_state* __restrict__ states,
                              unsigned long long* __restrict__ maps0,
                              unsigned long long* __restrict__ maps1,
-                             double* __restrict__ partials,
-                             int32_t* __restrict__ tickets) {
+                             double* __restrict__ partials) {
   __shared__ double sh[kShared];
-  __shared__ double tot[kSlots];
-  __shared__ int s_last;
   const PlanView pv = plan_view(plan);
   for (int t = blockIdx.x; t < pv.tiles; t += gridDim.x) {
     const int p = pv.tile_pair[t];
@@ -128,9 +126,7 @@ __global__ void __launch_bounds__(kBlock)
     const LevelView<S> L = view_of<S>(pv.levels[p]);
     const unsigned long long* map =
         pair_map(st, maps0, maps1, pv.map_off[p], 1);
-    const int first = pv.tile_start[p];
-    const int ntiles = pv.tile_start[p + 1] - first;
-    const int64_t base = int64_t(t - first) * pv.tile_px;
+    const int64_t base = int64_t(t - pv.tile_start[p]) * pv.tile_px;
     float m[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
@@ -141,25 +137,17 @@ __global__ void __launch_bounds__(kBlock)
       const int64_t idx = base + k;
       if (idx < L.n) information_pixel(L, m, map, idx, acc);
     }
-    block_partial(acc, partials + int64_t(t) * kSlots, sh, true);
-    if (arrive_last(&tickets[p], ntiles - 1, &s_last)) {
-      sum_partials<kBlock>(partials + int64_t(first) * kSlots, ntiles, sh, tot);
-      if (threadIdx.x == 0) tickets[p] = 0;
-      store_information(st, tot);
-    }
-    __syncthreads();
+    block_partial(acc, partials + int64_t(t) * kSlots, sh);
   }
 }
 
 __global__ void batch_begin_kernel(const void* __restrict__ plan,
                                    stb_icp_state* __restrict__ states,
-                                   const double* __restrict__ T0, int chain,
-                                   int32_t* __restrict__ tickets) {
+                                   const double* __restrict__ T0, int chain) {
   const PlanView pv = plan_view(plan);
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= pv.P) return;
   reset_state(states + p, T0 ? T0 + 16 * p : nullptr, chain);
-  tickets[p] = 0;
 }
 
 // Clears map entries [0, total) of both maps (spare == 0) or, per pair
@@ -249,8 +237,10 @@ void launch_batch(const void* plan, const stb_icp_batch_sizes& z,
                        plan, states, sc.map0, sc.map1, dthr2, cstar, fixed);
     if (phase == kInformation) {
       hipLaunchKernelGGL(batch_information_kernel<S>, dim3(gj), dim3(kBlock), 0,
-                         s, plan, states, sc.map0, sc.map1, sc.partials,
-                         sc.tickets);
+                         s, plan, states, sc.map0, sc.map1, sc.partials);
+      hipLaunchKernelGGL(batch_solve_kernel, dim3(z.num_pairs),
+                         dim3(kSolveBlock), 0, s, plan, states, sc.partials,
+                         0.0, 0.0, int(kStoreInformation), 1);
       continue;
     }
     const int solve = phase == kIterate ? 1 : 0;
@@ -386,7 +376,7 @@ extern "C" int stb_icp_batch_begin(const void* plan,
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(batch_begin_kernel, dim3((sizes->num_pairs + 63) / 64),
                      dim3(64), 0, s, plan, static_cast<stb_icp_state*>(states),
-                     transforms0, chain, sc.tickets);
+                     transforms0, chain);
   hipLaunchKernelGGL(batch_fill_kernel,
                      dim3(grid_for(sizes->total_map, kBlock, 4096)),
                      dim3(kBlock), 0, s, plan,

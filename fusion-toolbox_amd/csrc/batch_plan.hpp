@@ -59,17 +59,16 @@ __device__ inline PlanView plan_view(const void* plan) {
   return v;
 }
 
-// Scratch: map0[total_map] | map1[total_map] | partials[tiles][32] |
-// tickets[P].
+// Scratch: map0[total_map] | map1[total_map] | partials[tiles][32].
 struct Scratch {
   unsigned long long* map0;
   unsigned long long* map1;
   double* partials;
-  int32_t* tickets;
   size_t bytes;
 };
 
 inline Scratch carve_scratch(void* base, int P, int tiles, int64_t total_map) {
+  (void)P;
   Scratch s;
   char* p = static_cast<char*>(base);
   s.map0 = reinterpret_cast<unsigned long long*>(p);
@@ -78,8 +77,6 @@ inline Scratch carve_scratch(void* base, int P, int tiles, int64_t total_map) {
   p += align256(size_t(total_map) * 8);
   s.partials = reinterpret_cast<double*>(p);
   p += align256(size_t(tiles) * kSlots * sizeof(double));
-  s.tickets = reinterpret_cast<int32_t*>(p);
-  p += align256(size_t(P) * sizeof(int32_t));
   s.bytes = size_t(p - static_cast<char*>(base));
   return s;
 }

@@ -188,25 +188,27 @@ __global__ void __launch_bounds__(BLOCK)
     const int64_t hi = lo + chunk < map_len ? lo + chunk : map_len;
     for (int64_t i = lo + threadIdx.x; i < hi; i += BLOCK) next[i] = kKeySentinel;
   }
-  block_partial<BLOCK>(acc, partials + int64_t(blockIdx.x) * kSlots, sh, false);
+  block_partial<BLOCK>(acc, partials + int64_t(blockIdx.x) * kSlots, sh);
   // Last, so that its round trip does not delay block 0's pixels.
   if (rec && blockIdx.x == 0) commit_record(st, rec, tw.gw, tw.fw);
 }
 
-// The fixed-order sum of the Jacobian launch's `count` partials by one
-// block -- the order every correspondence launch that applies an update uses
-// too, so a loop and the split entry points give the same bits -- then the
-// pose update (do_solve) or the stored linearisation.
+// The fixed-order sum of `count` partials by one block -- the order every
+// correspondence launch that applies an update uses too, so a loop and the
+// split entry points give the same bits -- then the pose update, the stored
+// linearisation or the information matrix (`mode`, a FinishMode).
 __global__ void __launch_bounds__(kSolveBlock)
     solve_fused_kernel(stb_icp_state* __restrict__ st,
                        const double* __restrict__ partials, int count,
-                       double gw, double fw, int do_solve, int fixed_map) {
+                       double gw, double fw, int mode, int fixed_map) {
   __shared__ double sh[kSolveBlock];
   __shared__ double tot[kSlots];
   if (!fixed_map && st->stopped) return;
-  finish_partials<kSolveBlock>(st, partials, count, do_solve, gw, fw, sh, tot);
+  finish_partials<kSolveBlock>(st, partials, count, mode, gw, fw, sh, tot);
 }
 
+// G^T G of the matched pixels at the current pose over the spare map, one
+// FP64 partial per block; solve_fused_kernel (kStoreInformation) sums them.
 template <typename S>
 __global__ void __launch_bounds__(kBlock)
     information_fused_kernel(stb_icp_level Ld, stb_icp_state* __restrict__ st,
@@ -214,8 +216,6 @@ __global__ void __launch_bounds__(kBlock)
                              unsigned long long* __restrict__ map1,
                              double* __restrict__ partials) {
   __shared__ double sh[kShared];
-  __shared__ double tot[kSlots];
-  __shared__ int s_last;
   const unsigned long long* __restrict__ map = pick_map(st, map0, map1, 1);
   float m[12];
 #pragma unroll
@@ -227,11 +227,7 @@ __global__ void __launch_bounds__(kBlock)
   for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < L.n;
        idx += int64_t(gridDim.x) * blockDim.x)
     information_pixel(L, m, map, idx, acc);
-  block_partial(acc, partials + int64_t(blockIdx.x) * kSlots, sh, true);
-  if (!arrive_last(&st->ticket, int(gridDim.x) - 1, &s_last)) return;
-  sum_partials<kBlock>(partials, int(gridDim.x), sh, tot);
-  if (threadIdx.x == 0) st->ticket = 0;
-  store_information(st, tot);
+  block_partial(acc, partials + int64_t(blockIdx.x) * kSlots, sh);
 }
 
 __global__ void icp_begin_kernel(stb_icp_state* st,
@@ -273,7 +269,7 @@ __global__ void __launch_bounds__(kBlock)
 constexpr int kDefaultUpdateBlocks = 256;
 constexpr int64_t kWidePixels = 2 * 65536;
 constexpr int kMaxUpdateBlocks = 4096;
-constexpr int kInformationBlocks = 256;
+constexpr int kInformationBlocks = 1024;
 constexpr int kDefaultCorrespBlocks = 4096;
 // Blocks of the correspondence launch that also applies the previous update
 // (kSolveBlock threads each; every block sums the partials): one per CU.
@@ -445,12 +441,12 @@ void launch_phase(const stb_icp_level& L, const stb_icp_params& P,
   }
   if (phase == kCorrespond) return;
   if (phase == kInformation) {
-    // A one-off pass reduced by its last block: every block pays a fence and
-    // a ticket, so a few hundred blocks loop over the pixels.
-    hipLaunchKernelGGL(information_fused_kernel<S>,
-                       dim3(grid_for(L.n_src, kBlock, kInformationBlocks)),
-                       dim3(kBlock), 0, stream, L, w.st, w.map[0], w.map[1],
-                       w.partials);
+    const int grid = grid_for(L.n_src, kBlock, kInformationBlocks);
+    hipLaunchKernelGGL(information_fused_kernel<S>, dim3(grid), dim3(kBlock),
+                       0, stream, L, w.st, w.map[0], w.map[1], w.partials);
+    hipLaunchKernelGGL(solve_fused_kernel, dim3(1), dim3(kSolveBlock), 0,
+                       stream, w.st, w.partials, grid, 0.0, 0.0,
+                       int(kStoreInformation), 1);
     return;
   }
   const int solve = phase == kUpdate ? 1 : (phase == kJacobians ? -1 : 0);

@@ -6,9 +6,10 @@
 //   * the slot reduction: 21 unique JtJ + 6 Jr + r2/count bookkeeping per
 //     lane in FP32, recursive-halving across the wavefront (32 exchanges for
 //     32 slots), FP64 across waves through LDS, one FP64 partial per block,
-//     and a fixed-order FP64 sum of the partials by one block of a following
-//     launch (or, for one-off passes, by the last block to arrive);
-//   * the per-pair state update: FP64 Cholesky solve and exp_rt_to_matrix.
+//     and a fixed-order FP64 sum of the partials in a following launch (by
+//     one block, or by every block of a loop's next correspondence launch);
+//   * the per-pair state update: FP64 Cholesky solve and exp_rt_to_matrix,
+//     in one thread or spread over one wavefront.
 #pragma once
 
 #include "icp_math.hpp"
@@ -280,14 +281,13 @@ __device__ inline int bitrev5(int l) {
          ((l & 16) >> 4);
 }
 
-// Reduces the block's slots and stores them in FP64 at `out` (32 doubles).
-// With `publish` the stores are fenced device-wide, for a consumer in the
-// same launch (arrive_last); a consumer in a later launch needs no fence, and
-// each fence costs a few nanoseconds per block at the memory side.
-// `sh` is LDS of BLOCK / 64 * kSlots doubles (kShared for kBlock).
+// Reduces the block's slots and stores them in FP64 at `out` (32 doubles),
+// with no fence: every consumer runs in a later launch (a fence and a ticket
+// for a consumer in the same launch cost each block ~20 ns at the memory
+// side). `sh` is LDS of BLOCK / 64 * kSlots doubles (kShared for kBlock).
 template <int BLOCK = kBlock>
 __device__ inline void block_partial(float (&acc)[kSlots], double* out,
-                                     double* sh, bool publish) {
+                                     double* sh) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   wave_reduce_slots(acc, lane);
@@ -298,20 +298,8 @@ __device__ inline void block_partial(float (&acc)[kSlots], double* out,
 #pragma unroll
     for (int w = 0; w < BLOCK / 64; ++w) s += sh[w * kSlots + threadIdx.x];
     out[threadIdx.x] = s;
-    if (publish) __threadfence();
   }
   __syncthreads();
-}
-
-// Arrival ticket: true in every thread of the block that brings `counter` to
-// `expected` (the last of `expected` + 1 arrivals).
-__device__ inline bool arrive_last(int32_t* counter, int expected,
-                                   int* s_flag) {
-  if (threadIdx.x == 0) *s_flag = (atomicAdd(counter, 1) == expected);
-  __syncthreads();
-  const bool last = *s_flag != 0;
-  if (last) __threadfence();
-  return last;
 }
 
 // Fixed-order FP64 sum of `count` partials (32 doubles each) into tot[32] by
@@ -601,26 +589,33 @@ __device__ inline void store_information(stb_icp_state* st, const double* tot) {
   if (t < 36) st->information[t] = sym6(tot, t / 6, t % 6);
 }
 
+// What finish_partials does with the summed partials.
+enum FinishMode { kStoreLinearization = 0, kUpdatePose = 1, kStoreInformation = 2 };
+
 // Sums `count` partials and applies them to the state: a Gauss-Newton update
-// (do_solve) or only the stored linearisation. Called by every thread of one
-// block of BLOCK threads; `sh` is LDS of BLOCK doubles, `tot` of kSlots.
+// (kUpdatePose), the stored linearisation, or the information matrix (the
+// partials of information_pixel). Called by every thread of one block of
+// BLOCK threads; `sh` is LDS of BLOCK doubles, `tot` of kSlots.
 template <int BLOCK>
 __device__ inline void finish_partials(stb_icp_state* st, const double* partials,
-                                       int count, int do_solve, double gw,
+                                       int count, int mode, double gw,
                                        double fw, double* sh, double* tot) {
   const int t = threadIdx.x;
+  const bool update = mode == kUpdatePose;
   // The state loads of the update issue before the partial loads.
   double row = 0.0;
   int iters = 0, sel = 0;
-  if (do_solve && t < 12) row = st->transform[t];
-  if (do_solve && t == 0) {
+  if (update && t < 12) row = st->transform[t];
+  if (update && t == 0) {
     iters = st->iterations;
     sel = st->map_sel;
   }
   sum_partials<BLOCK>(partials, count, sh, tot);
-  if (do_solve) {
+  if (update) {
     if (t < 12) sh[t] = row;
     gn_finalize(st, tot, gw, fw, sh, iters, sel);
+  } else if (mode == kStoreInformation) {
+    store_information(st, tot);
   } else {
     store_linearization(st, tot);
   }

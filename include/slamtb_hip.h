@@ -377,7 +377,7 @@ typedef struct stb_icp_state {
   float nm[9];            /* inverse-transpose of rt's rotation */
   int32_t stopped;        /* 1 after a failed Cholesky (the loop's `break`) */
   int32_t iterations;     /* accepted iterations */
-  int32_t ticket;         /* block-arrival counter of the reduction kernels */
+  int32_t ticket;         /* reserved, zero */
   int32_t map_sel;        /* correspondence map the next iteration uses */
 } stb_icp_state;
 
@@ -457,7 +457,7 @@ typedef struct stb_icp_batch_sizes {
   int32_t storage;      /* storage dtype shared by every pair */
   int64_t total_map;    /* sum of target pixels over the pairs */
   size_t plan_bytes;    /* size of the plan image */
-  size_t scratch_bytes; /* device scratch: two maps per pair, partials, tickets */
+  size_t scratch_bytes; /* device scratch: two maps per pair, tile partials */
   int32_t max_channels; /* most feature channels of any pair */
   int32_t reserved;
 } stb_icp_batch_sizes;
