@@ -328,6 +328,9 @@ extern "C" int stb_icp_batch_plan(int32_t num_pairs,
                 "stb_icp_batch_plan: pair %d is empty", p);
     STB_REQUIRE(L.storage == storage,
                 "stb_icp_batch_plan: all pairs must share one storage type");
+    STB_REQUIRE(L.channels == 0 || feats_addressable(L),
+                "stb_icp_batch_plan: pair %d's feature maps must each hold "
+                "fewer than 2^31 bytes", p);
     start[p] = int32_t(tiles);
     off[p] = maps;
     tiles += (L.n_src + tile_px - 1) / tile_px;

@@ -356,6 +356,8 @@ int check_level(const stb_icp_level* L, bool need_feats) {
   if (need_feats) {
     STB_REQUIRE(L->src_feats && L->tgt_feats && L->channels > 0,
                 "feature term needs src_feats, tgt_feats and channels > 0");
+    STB_REQUIRE(feats_addressable(*L),
+                "feature maps must each hold fewer than 2^31 bytes");
   }
   return STB_OK;
 }

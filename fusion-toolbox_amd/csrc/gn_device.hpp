@@ -163,6 +163,140 @@ __device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
 // loads issue in one batch: a pixel costs two memory round trips. A pixel
 // that does not own its target adds exact zeros (selected, never multiplied
 // in, so NaN or inf in its unused data cannot leak into the sums).
+// ---------------------------------------------------------------------------
+// Feature taps through buffer resources
+// ---------------------------------------------------------------------------
+//
+// The fused kernels read the feature maps through raw buffer resources over
+// the whole [C, H, W] target and [C, n] source arrays: a tap is a 32-bit
+// byte offset (the channel's in a scalar register, the texel's in a vector
+// register, the +1 column step as an immediate) instead of a 64-bit address
+// of its own, and a load past the end of the array returns 0 instead of
+// faulting. The bilinear taps therefore need no clamping: a tap past the
+// right edge reads the next row's first texel, one past the bottom the next
+// channel's (or 0 past the last), and bil_mix replaces both by zero exactly
+// as it replaces the clamped re-reads of bil_taps. Both arrays must hold
+// fewer than 2^31 bytes (checked on the host).
+constexpr int kRawBufferWord3 = 0x00020000;  // raw 32-bit data format, CDNA
+
+// Whether a level's feature arrays fit those 32-bit byte offsets.
+inline bool feats_addressable(const stb_icp_level& L) {
+  const int64_t es = L.storage == STB_F32 ? 4 : 2;
+  const int64_t lim = (int64_t(1) << 31) - 1;
+  return int64_t(L.channels) * L.height * L.width * es <= lim &&
+         int64_t(L.channels) * L.n_src * es <= lim;
+}
+
+template <typename S>
+__device__ inline __amdgpu_buffer_rsrc_t make_res(const S* p, int64_t elems) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<S*>(p), short(0),
+                                           int(elems * int64_t(sizeof(S))),
+                                           kRawBufferWord3);
+}
+
+template <typename S>
+__device__ inline float res_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ inline float res_ld<float>(__amdgpu_buffer_rsrc_t r, int voff,
+                                      int soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ inline float res_ld<__hip_bfloat16>(__amdgpu_buffer_rsrc_t r,
+                                               int voff, int soff) {
+  return bf16_bits_to_float(__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+}
+template <>
+__device__ inline float res_ld<__half>(__amdgpu_buffer_rsrc_t r, int voff,
+                                       int soff) {
+  const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+  return float(__builtin_bit_cast(_Float16, b));
+}
+
+template <typename S>
+__device__ inline void res_taps(__amdgpu_buffer_rsrc_t r, int soff, int W,
+                                const Bil<float>& b, float t[4]) {
+  constexpr int es = int(sizeof(S));
+  const int o = (b.vi * W + b.ui) * es;
+  const int d = o + W * es;
+  t[0] = res_ld<S>(r, o, soff);
+  t[1] = res_ld<S>(r, o + es, soff);
+  t[2] = res_ld<S>(r, d, soff);
+  t[3] = res_ld<S>(r, d + es, soff);
+}
+
+// feature_terms<float, S, S, true, NB> (icp_math.hpp) with its taps and
+// source values read through buffer resources: the same values in the same
+// arithmetic, so the same bits.
+template <typename S, int NB>
+__device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
+                                          const V3<float>& p, float u, float v,
+                                          float j[6], float& rlast) {
+#pragma clang fp contract(off)
+  constexpr int es = int(sizeof(S));
+  const __amdgpu_buffer_rsrc_t rt = make_res(L.tf, int64_t(L.C) * L.HW);
+  const __amdgpu_buffer_rsrc_t rs = make_res(L.sf, int64_t(L.C) * L.n);
+  const int tch = int(L.HW) * es;  // bytes per target channel
+  const int sch = int(L.n) * es;   // bytes per source channel
+  const int so = int(idx) * es;
+  const int H = L.H, W = L.W, C = L.C;
+  const float h = 0.05f;
+  const float div = 1.0f / (2.0f * h);
+  const Bil<float> c0 = make_bil(u, v);
+  const Bil<float> bu0 = make_bil(u + h, v);
+  const Bil<float> bu1 = make_bil(fmax(u - h, 0.0f), v);
+  const Bil<float> bv0 = make_bil(u, v + h);
+  const Bil<float> bv1 = make_bil(u, fmax(v - h, 0.0f));
+  // The last channel's five samples first (see feature_terms).
+  const int lt = (C - 1) * tch;
+  float t0[4], t1[4], t2[4], t3[4], t4[4];
+  res_taps<S>(rt, lt, W, c0, t0);
+  res_taps<S>(rt, lt, W, bu0, t1);
+  res_taps<S>(rt, lt, W, bu1, t2);
+  res_taps<S>(rt, lt, W, bv0, t3);
+  res_taps<S>(rt, lt, W, bv1, t4);
+  const float flast = res_ld<S>(rs, so, (C - 1) * sch);
+  const float last = bil_mix(H, W, c0, t0);
+  const float su0 = bil_mix(H, W, bu0, t1), su1 = bil_mix(H, W, bu1, t2);
+  const float sv0 = bil_mix(H, W, bv0, t3), sv1 = bil_mix(H, W, bv1, t4);
+  float sq = 0.0f;
+  const int last_c = C - 2 > 0 ? C - 2 : 0;
+  for (int c0i = 0; c0i < C - 1; c0i += NB) {
+    float taps[NB][4], fs[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const int c = c0i + k < last_c ? c0i + k : last_c;
+      res_taps<S>(rt, c * tch, W, c0, taps[k]);
+      fs[k] = res_ld<S>(rs, so, c * sch);
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const float res = bil_mix(H, W, c0, taps[k]) - fs[k];
+      sq += (c0i + k < C - 1) ? res * res : 0.0f;
+    }
+  }
+  const float res = last - flast;
+  const float du = (su0 - su1) * div;
+  const float dv = (sv0 - sv1) * div;
+  const float iz = __builtin_amdgcn_rcpf(p.z);
+  const float iz2 = iz * iz;
+  const float j00 = L.fx * iz;
+  const float j02 = -p.x * L.fx * iz2;
+  const float j11 = L.fy * iz;
+  const float j12 = -p.y * L.fy * iz2;
+  const V3<float> g{du * j00, dv * j11, du * j02 + dv * j12};
+  const V3<float> c = cross3(p, g);
+  j[0] = g.x;
+  j[1] = g.y;
+  j[2] = g.z;
+  j[3] = c.x;
+  j[4] = c.y;
+  j[5] = c.z;
+  rlast = res;
+  sq += res * res;
+  return sq;
+}
+
 // With `clear` (a Gauss-Newton loop's own map) the owner of a map entry
 // resets it to the sentinel after reading it: every entry the
 // correspondence launch set has an owner that projects onto it again here,
@@ -206,8 +340,7 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
   }
   if (FEAT) {
     float jf[6], rl;
-    const float sq = feature_terms<float, S, S, true, NB>(
-        L.tf, L.C, L.H, L.W, L.HW, L.sf, L.n, idx, L.fx, L.fy, p, u, v, jf, rl);
+    const float sq = feature_terms_res<S, NB>(L, idx, p, u, v, jf, rl);
     const bool use = own && !(sq > tw.fthr2);
     float j[6];
 #pragma unroll
