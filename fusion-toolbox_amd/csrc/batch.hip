@@ -31,6 +31,7 @@ __global__ void __launch_bounds__(kBlock)
     const stb_icp_state* st = states + p;
     if (!fixed_map && st->stopped) continue;
     const LevelView<S> L = view_of<S>(pv.levels[p]);
+    const LevelRes R = res_of(L);
     float m[12], nm[9];
 #pragma unroll
     for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
@@ -41,7 +42,7 @@ __global__ void __launch_bounds__(kBlock)
     const int64_t base = int64_t(t - pv.tile_start[p]) * pv.tile_px;
     for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
       const int64_t idx = base + k;
-      if (idx < L.n) corresp_pixel(L, m, nm, idx, map, dthr2, cstar);
+      if (idx < L.n) corresp_pixel(L, R, m, nm, idx, map, dthr2, cstar);
     }
   }
 }
@@ -61,6 +62,7 @@ __global__ void __launch_bounds__(kBlock)
     stb_icp_state* st = states + p;
     if (!fixed_map && st->stopped) continue;
     const LevelView<S> L = view_of<S>(pv.levels[p]);
+    const LevelRes R = res_of(L);
     const int64_t off = pv.map_off[p];
     const unsigned long long* map = pair_map(st, maps0, maps1, off, fixed_map);
     const int first = pv.tile_start[p];
@@ -85,7 +87,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
     for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
       const int64_t idx = base + k;
-      if (idx < L.n) jacobian_pixel<S, GEOM, FEAT, NB>(L, m, map, idx, tw, acc);
+      if (idx < L.n) jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, idx, tw, acc);
     }
     block_partial(acc, partials + int64_t(t) * kSlots, sh);
   }
@@ -328,9 +330,9 @@ extern "C" int stb_icp_batch_plan(int32_t num_pairs,
                 "stb_icp_batch_plan: pair %d is empty", p);
     STB_REQUIRE(L.storage == storage,
                 "stb_icp_batch_plan: all pairs must share one storage type");
-    STB_REQUIRE(L.channels == 0 || feats_addressable(L),
-                "stb_icp_batch_plan: pair %d's feature maps must each hold "
-                "fewer than 2^31 bytes", p);
+    STB_REQUIRE(level_addressable(L),
+                "stb_icp_batch_plan: every array of pair %d must hold fewer "
+                "than 2^31 bytes", p);
     start[p] = int32_t(tiles);
     off[p] = maps;
     tiles += (L.n_src + tile_px - 1) / tile_px;

@@ -41,9 +41,10 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int i = 0; i < 9; ++i) nm[i] = st->nm[i];
   const LevelView<S> L = view_of<S>(Ld);
+  const LevelRes R = res_of(L);
   for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < L.n;
        idx += int64_t(gridDim.x) * blockDim.x)
-    corresp_pixel(L, m, nm, idx, map, dthr2, cstar);
+    corresp_pixel(L, R, m, nm, idx, map, dthr2, cstar);
 }
 
 // The correspondence launch of iteration k > 0 of a loop: every block first
@@ -76,11 +77,12 @@ __global__ void __launch_bounds__(kSolveBlock)
   const int64_t lo = int64_t(blockIdx.x) * chunk;
   const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
   const int64_t idx = lo + t;
-  // Unconditional (index clamped) so that no branch waits for them.
-  const int64_t ic = idx < L.n ? idx : L.n - 1;
-  const uint8_t vs = L.sm[ic];
-  const V3<float> ps = ld3f(L.sp, ic);
-  const V3<float> ns = ld3f(L.sn, ic);
+  // Unconditional, so that no branch waits for them (past the end of the
+  // arrays the buffer loads return 0).
+  const LevelRes R = res_of(L);
+  const uint8_t vs = res_u8(R.sm, int(idx));
+  const V3<float> ps = res_ld3<S>(R.sp, int(idx));
+  const V3<float> ns = res_ld3<S>(R.sn, int(idx));
   const double row = st->transform[t < 12 ? t : 0];
   // The flags stopped, iterations, ticket, map_sel (consecutive words) in
   // one vector load, lane l of each wavefront taking word l % 4: a uniform
@@ -126,9 +128,9 @@ __global__ void __launch_bounds__(kSolveBlock)
   }
   unsigned long long* __restrict__ map = use_sel ? map1 : map0;
   if (idx < hi)
-    corresp_pixel_at(L, r.m, r.nm, idx, vs, ps, ns, map, dthr2, cstar);
+    corresp_pixel_at(L, R, r.m, r.nm, idx, vs, ps, ns, map, dthr2, cstar);
   for (int64_t i = idx + kSolveBlock; i < hi; i += kSolveBlock)
-    corresp_pixel(L, r.m, r.nm, i, map, dthr2, cstar);
+    corresp_pixel(L, R, r.m, r.nm, i, map, dthr2, cstar);
 }
 
 // Per-pixel Jacobians of one iteration; every block leaves one FP64 partial
@@ -161,12 +163,13 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
   for (int i = 0; i < 12; ++i) m[i] = rt[i];
   const LevelView<S> L = view_of<S>(Ld);
+  const LevelRes R = res_of(L);
   float acc[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
   if (ONE_PX) {
     if (gtid < L.n)
-      jacobian_pixel<S, GEOM, FEAT, NB>(L, m, map, gtid, tw, acc, clear);
+      jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, gtid, tw, acc, clear);
   } else {
     // Each block takes a contiguous run of pixels, so the pixels beyond one
     // per thread spread over every block instead of doubling the work of the
@@ -176,7 +179,7 @@ __global__ void __launch_bounds__(BLOCK)
     const int64_t lo = int64_t(blockIdx.x) * chunk;
     const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
     for (int64_t idx = lo + threadIdx.x; idx < hi; idx += BLOCK)
-      jacobian_pixel<S, GEOM, FEAT, NB>(L, m, map, idx, tw, acc, clear);
+      jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, idx, tw, acc, clear);
   }
   if (!fixed_map && rec == nullptr) {
     // The first iteration of a loop (or a split update) also sweeps the
@@ -353,11 +356,12 @@ int check_level(const stb_icp_level* L, bool need_feats) {
   STB_REQUIRE(L->storage == STB_F32 || L->storage == STB_BF16 ||
                   L->storage == STB_F16,
               "storage must be STB_F32, STB_BF16 or STB_F16");
+  STB_REQUIRE(level_addressable(*L),
+              "every level array (points, normals, masks, features, map) must "
+              "hold fewer than 2^31 bytes");
   if (need_feats) {
     STB_REQUIRE(L->src_feats && L->tgt_feats && L->channels > 0,
                 "feature term needs src_feats, tgt_feats and channels > 0");
-    STB_REQUIRE(feats_addressable(*L),
-                "feature maps must each hold fewer than 2^31 bytes");
   }
   return STB_OK;
 }

@@ -89,102 +89,32 @@ __device__ inline LevelView<S> view_of(const stb_icp_level& L) {
   return v;
 }
 
-// Correspondence of source pixel idx (src/correspondence_map.cu:38-61 with
-// RobustCorrespondence::Match): transform, project, round, bounds, target
-// mask, distance and normal-angle gates, then a 64-bit unsigned atomic min of
-// (bits(distance) << 32 | idx) into the map. Each test is written after the
-// loads it depends on and reads their results, so the compiler cannot sink a
-// load below an earlier test: the source loads issue together, then the
-// target loads, and a pixel costs two memory round trips.
-// corresp_pixel from the pixel's already loaded source mask, point and
-// normal.
-template <typename S>
-__device__ inline void corresp_pixel_at(const LevelView<S>& L, const float* m,
-                                        const float* nm, int64_t idx,
-                                        uint8_t valid_src, V3<float> ps,
-                                        V3<float> ns,
-                                        unsigned long long* __restrict__ map,
-                                        float dthr2, float cstar) {
-#pragma clang fp contract(off)
-  const V3<float> p = rigid_point(m, ps);
-  const V3<float> nn = rigid_normal(nm, ns);
-  float u, v;
-  project(L.fx, L.fy, L.cx, L.cy, p, u, v);
-  int ui, vi;
-  const bool in_u = round_in_range(u, L.W, ui);
-  const bool in_v = round_in_range(v, L.H, vi);
-  if (!valid_src || !in_u || !in_v) return;
-  const int64_t pos = int64_t(vi) * L.W + ui;
-  const uint8_t valid = L.tm[pos];
-  const V3<float> q = ld3f(L.tp, pos);
-  const V3<float> qn = ld3f(L.tn, pos);
-  const V3<float> d = sub3(q, p);
-  const float sq = dot3(d, d);
-  const bool keep = (valid != 0) & !(sq > dthr2) &
-                    !angle_rejects(gate_cos(nn, qn), cstar);
-  if (!keep) return;
-  atomicMin(map + pos, (unsigned long long)make_key(sqrtf(sq), uint32_t(idx)));
-}
-
-template <typename S>
-__device__ inline void corresp_pixel(const LevelView<S>& L, const float* m,
-                                     const float* nm, int64_t idx,
-                                     unsigned long long* __restrict__ map,
-                                     float dthr2, float cstar) {
-  const uint8_t valid_src = L.sm[idx];
-  const V3<float> ps = ld3f(L.sp, idx);
-  const V3<float> ns = ld3f(L.sn, idx);
-  corresp_pixel_at(L, m, nm, idx, valid_src, ps, ns, map, dthr2, cstar);
-}
-
-// Adds one weighted term J' = w J, r' (already scaled) to the JtJ/Jr slots.
-__device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
-                                float r) {
-  float jw[6];
-#pragma unroll
-  for (int a = 0; a < 6; ++a) jw[a] = j[a] * w;
-  int k = 0;
-#pragma unroll
-  for (int a = 0; a < 6; ++a)
-#pragma unroll
-    for (int b = a; b < 6; ++b) {
-      acc[k] = fmaf(jw[a], jw[b], acc[k]);
-      ++k;
-    }
-#pragma unroll
-  for (int a = 0; a < 6; ++a) acc[21 + a] = fmaf(jw[a], r, acc[21 + a]);
-}
-
-// Point-to-plane and feature terms of source pixel idx when the map names it
-// as its target pixel's owner (src/icp_jacobian_geometric.cu:54-85,
-// src/icp_jacobian_feature.cu:58-109). The owner word, the target point and
-// normal and the feature taps around (u, v) are loaded whatever the owner
-// test says and the terms are accumulated through selects, so all of those
-// loads issue in one batch: a pixel costs two memory round trips. A pixel
-// that does not own its target adds exact zeros (selected, never multiplied
-// in, so NaN or inf in its unused data cannot leak into the sums).
 // ---------------------------------------------------------------------------
-// Feature taps through buffer resources
+// Gathers through buffer resources
 // ---------------------------------------------------------------------------
 //
-// The fused kernels read the feature maps through raw buffer resources over
-// the whole [C, H, W] target and [C, n] source arrays: a tap is a 32-bit
-// byte offset (the channel's in a scalar register, the texel's in a vector
-// register, the +1 column step as an immediate) instead of a 64-bit address
-// of its own, and a load past the end of the array returns 0 instead of
-// faulting. The bilinear taps therefore need no clamping: a tap past the
-// right edge reads the next row's first texel, one past the bottom the next
-// channel's (or 0 past the last), and bil_mix replaces both by zero exactly
-// as it replaces the clamped re-reads of bil_taps. Both arrays must hold
-// fewer than 2^31 bytes (checked on the host).
+// The fused correspondence and Jacobian passes read their inputs through
+// raw buffer resources over whole arrays (points, normals and masks of both
+// frames, the [C, H, W] target and [C, n] source features, the map): a load
+// is a 32-bit byte offset (an array's channel in a scalar register, the
+// element in a vector register, small constant steps as immediates) instead
+// of a 64-bit address of its own, and a load past the end of the array
+// returns 0 instead of faulting. The bilinear taps therefore need no
+// clamping: a tap past the right edge reads the next row's first texel, one
+// past the bottom the next channel's (or 0 past the last), and bil_mix
+// replaces both by zero exactly as it replaces the clamped re-reads of
+// bil_taps. Every array must hold fewer than 2^31 bytes (checked on the
+// host).
 constexpr int kRawBufferWord3 = 0x00020000;  // raw 32-bit data format, CDNA
 
-// Whether a level's feature arrays fit those 32-bit byte offsets.
-inline bool feats_addressable(const stb_icp_level& L) {
+// Whether a level's arrays fit those 32-bit byte offsets.
+inline bool level_addressable(const stb_icp_level& L) {
   const int64_t es = L.storage == STB_F32 ? 4 : 2;
   const int64_t lim = (int64_t(1) << 31) - 1;
-  return int64_t(L.channels) * L.height * L.width * es <= lim &&
-         int64_t(L.channels) * L.n_src * es <= lim;
+  const int64_t hw = int64_t(L.height) * L.width;
+  return int64_t(L.channels) * hw * es <= lim &&
+         int64_t(L.channels) * L.n_src * es <= lim && 3 * hw * es <= lim &&
+         3 * L.n_src * es <= lim && 8 * hw <= lim;
 }
 
 template <typename S>
@@ -213,6 +143,112 @@ __device__ inline float res_ld<__half>(__amdgpu_buffer_rsrc_t r, int voff,
   return float(__builtin_bit_cast(_Float16, b));
 }
 
+__device__ inline uint8_t res_u8(__amdgpu_buffer_rsrc_t r, int voff) {
+  return __builtin_amdgcn_raw_buffer_load_b8(r, voff, 0, 0);
+}
+
+// Element i of an [m, 3] array.
+template <typename S>
+__device__ inline V3<float> res_ld3(__amdgpu_buffer_rsrc_t r, int i) {
+  constexpr int es = int(sizeof(S));
+  const int o = 3 * es * i;
+  return V3<float>{res_ld<S>(r, o, 0), res_ld<S>(r, o + es, 0),
+                   res_ld<S>(r, o + 2 * es, 0)};
+}
+
+// Buffer resources over a level's frame arrays.
+struct LevelRes {
+  __amdgpu_buffer_rsrc_t sp, sn, sm, tp, tn, tm;
+};
+
+template <typename S>
+__device__ inline LevelRes res_of(const LevelView<S>& L) {
+  LevelRes r;
+  r.sp = make_res(L.sp, 3 * L.n);
+  r.sn = make_res(L.sn, 3 * L.n);
+  r.sm = make_res(L.sm, L.n);
+  r.tp = make_res(L.tp, 3 * L.HW);
+  r.tn = make_res(L.tn, 3 * L.HW);
+  r.tm = make_res(L.tm, L.HW);
+  return r;
+}
+
+// Correspondence of source pixel idx (src/correspondence_map.cu:38-61 with
+// RobustCorrespondence::Match): transform, project, round, bounds, target
+// mask, distance and normal-angle gates, then a 64-bit unsigned atomic min of
+// (bits(distance) << 32 | idx) into the map. Each test is written after the
+// loads it depends on and reads their results, so the compiler cannot sink a
+// load below an earlier test: the source loads issue together, then the
+// target loads, and a pixel costs two memory round trips.
+// corresp_pixel from the pixel's already loaded source mask, point and
+// normal.
+template <typename S>
+__device__ inline void corresp_pixel_at(const LevelView<S>& L,
+                                        const LevelRes& R, const float* m,
+                                        const float* nm, int64_t idx,
+                                        uint8_t valid_src, V3<float> ps,
+                                        V3<float> ns,
+                                        unsigned long long* __restrict__ map,
+                                        float dthr2, float cstar) {
+#pragma clang fp contract(off)
+  const V3<float> p = rigid_point(m, ps);
+  const V3<float> nn = rigid_normal(nm, ns);
+  float u, v;
+  project(L.fx, L.fy, L.cx, L.cy, p, u, v);
+  int ui, vi;
+  const bool in_u = round_in_range(u, L.W, ui);
+  const bool in_v = round_in_range(v, L.H, vi);
+  if (!valid_src || !in_u || !in_v) return;
+  const int pos = vi * L.W + ui;
+  const uint8_t valid = res_u8(R.tm, pos);
+  const V3<float> q = res_ld3<S>(R.tp, pos);
+  const V3<float> qn = res_ld3<S>(R.tn, pos);
+  const V3<float> d = sub3(q, p);
+  const float sq = dot3(d, d);
+  const bool keep = (valid != 0) & !(sq > dthr2) &
+                    !angle_rejects(gate_cos(nn, qn), cstar);
+  if (!keep) return;
+  atomicMin(map + pos, (unsigned long long)make_key(sqrtf(sq), uint32_t(idx)));
+}
+
+template <typename S>
+__device__ inline void corresp_pixel(const LevelView<S>& L, const LevelRes& R,
+                                     const float* m, const float* nm,
+                                     int64_t idx,
+                                     unsigned long long* __restrict__ map,
+                                     float dthr2, float cstar) {
+  const uint8_t valid_src = res_u8(R.sm, int(idx));
+  const V3<float> ps = res_ld3<S>(R.sp, int(idx));
+  const V3<float> ns = res_ld3<S>(R.sn, int(idx));
+  corresp_pixel_at(L, R, m, nm, idx, valid_src, ps, ns, map, dthr2, cstar);
+}
+
+// Adds one weighted term J' = w J, r' (already scaled) to the JtJ/Jr slots.
+__device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
+                                float r) {
+  float jw[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) jw[a] = j[a] * w;
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = a; b < 6; ++b) {
+      acc[k] = fmaf(jw[a], jw[b], acc[k]);
+      ++k;
+    }
+#pragma unroll
+  for (int a = 0; a < 6; ++a) acc[21 + a] = fmaf(jw[a], r, acc[21 + a]);
+}
+
+// Point-to-plane and feature terms of source pixel idx when the map names it
+// as its target pixel's owner (src/icp_jacobian_geometric.cu:54-85,
+// src/icp_jacobian_feature.cu:58-109). The owner word, the target point and
+// normal and the feature taps around (u, v) are loaded whatever the owner
+// test says and the terms are accumulated through selects, so all of those
+// loads issue in one batch: a pixel costs two memory round trips. A pixel
+// that does not own its target adds exact zeros (selected, never multiplied
+// in, so NaN or inf in its unused data cannot leak into the sums).
 template <typename S>
 __device__ inline void res_taps(__amdgpu_buffer_rsrc_t r, int soff, int W,
                                 const Bil<float>& b, float t[4]) {
@@ -306,13 +342,14 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
 // the pixel has issued: the map is not a restrict pointer (it is written),
 // so no later load could move above the store.
 template <typename S, bool GEOM, bool FEAT, int NB = 4>
-__device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
+__device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
+                                      const float* m,
                                       const unsigned long long* map,
                                       int64_t idx, const TermWeights& tw,
                                       float (&acc)[kSlots], bool clear = false) {
 #pragma clang fp contract(off)
-  const uint8_t valid_src = L.sm[idx];
-  const V3<float> ps = ld3f(L.sp, idx);
+  const uint8_t valid_src = res_u8(R.sm, int(idx));
+  const V3<float> ps = res_ld3<S>(R.sp, int(idx));
   const V3<float> p = rigid_point(m, ps);
   float u, v;
   project(L.fx, L.fy, L.cx, L.cy, p, u, v);
@@ -320,15 +357,16 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const float* m,
   const bool in_u = round_in_range(u, L.W, ui);
   const bool in_v = round_in_range(v, L.H, vi);
   if (!valid_src || !in_u || !in_v) return;
-  const int64_t pos = int64_t(vi) * L.W + ui;
+  const int pos = vi * L.W + ui;
   // Only the owner half of the key (the low word: idx) is read. Loading the
   // whole key would leave its unused high half's register free for reuse,
   // which makes the compiler wait for the load before anything else issues.
-  const uint32_t owner = reinterpret_cast<const uint32_t*>(map)[2 * pos];
+  const __amdgpu_buffer_rsrc_t rmap = make_res(map, L.HW);
+  const uint32_t owner = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos, 0, 0);
   const bool own = owner == uint32_t(idx);
   if (GEOM) {
-    const V3<float> q = ld3f(L.tp, pos);
-    const V3<float> qn = ld3f(L.tn, pos);
+    const V3<float> q = res_ld3<S>(R.tp, pos);
+    const V3<float> qn = res_ld3<S>(R.tn, pos);
     const float r = dot3(sub3(q, p), qn);
     const V3<float> c = cross3(p, qn);
     const float j[6] = {own ? qn.x : 0.0f, own ? qn.y : 0.0f, own ? qn.z : 0.0f,
