@@ -276,7 +276,8 @@ def midstop_pair():
     return P, synthetic.random_se3(rng, 30, 0.5)
 
 
-@pytest.mark.parametrize("case", ["geom320", "both640", "midstop"])
+@pytest.mark.parametrize("case", ["geom320", "both640", "midstop", "odd1000x700",
+                                  "bf16_640"])
 def test_loop_equals_split_iterations(case):
     """stb_icp_iterate applies each update inside the next iteration's
     correspondence launch (every block sums the partials and solves; the
@@ -284,19 +285,26 @@ def test_loop_equals_split_iterations(case):
     stb_icp_update apply it in a one-block launch with the one-thread
     factorisation. Same sums, same subtractions in the same order: the two
     leave bit-identical states, also when the loop stops part-way."""
-    gw, fw, dthr, iters = 1.0, 0.0, 0.1, 10
+    gw, fw, dthr, iters, storage = 1.0, 0.0, 0.1, 10, None
     if case == "geom320":
         P, T0 = prepared_pair(9, 320, 240), small_motion()
     elif case == "both640":  # 1024-thread Jacobian blocks
         P, T0 = prepared_pair(7, 640, 480, channels=3), small_motion(5)
         gw, fw, iters = 10.0, 1.0, 5
+    elif case == "odd1000x700":  # uneven runs: ~2.7 pixels per thread
+        P, T0 = prepared_pair(13, 1000, 700, channels=5), small_motion(6)
+        gw, fw, iters = 10.0, 1.0, 4
+    elif case == "bf16_640":
+        P, T0 = prepared_pair(7, 640, 480, channels=3), small_motion(5)
+        gw, fw, iters, storage = 10.0, 1.0, 4, torch.bfloat16
     else:
         P, T0 = midstop_pair()
         dthr = 0.05
     g = to_dev(P, DEV)
     level = Level(P["kcam"], g["src_pts"], g["src_nrm"], g["src_mask"],
                   g["tgt_pts"], g["tgt_nrm"], g["tgt_mask"],
-                  g["src_feats"] if fw else None, g["tgt_feats"] if fw else None)
+                  g["src_feats"] if fw else None, g["tgt_feats"] if fw else None,
+                  storage=storage)
     params = make_params(gw, fw, dthr, ANG20, 2.75, fw > 0)
     run = runner_for(DEV)
     run.begin(level, torch.from_numpy(T0))
