@@ -399,14 +399,17 @@ int stb_icp_begin(void* workspace, size_t workspace_bytes,
                   const double* transform0, int32_t chain, int64_t n_src,
                   int32_t height, int32_t width, void* stream);
 
-/* Enqueues `num_iters` Gauss-Newton iterations. Each iteration runs a
- * correspondence kernel and a fused Jacobian/reduction kernel whose last
- * block sums the per-block partials in FP64 in a fixed order, solves the
- * 6x6 system by Cholesky in FP64 and left-multiplies exp_rt_to_matrix(x)
- * into the pose. A failed factorisation sets `stopped`, after which the
- * remaining iterations do nothing. Which of the two correspondence maps an
- * iteration uses is kept in the state, so a captured graph of any number of
- * iterations replays correctly. */
+/* Enqueues `num_iters` Gauss-Newton iterations in 2 * num_iters + 1
+ * launches. Each iteration is a correspondence launch and a Jacobian launch
+ * that leaves one FP64 partial per block; the partials are summed in a fixed
+ * order, the 6x6 system solved by Cholesky in FP64 and exp_rt_to_matrix(x)
+ * left-multiplied into the pose by the next iteration's correspondence
+ * launch (and, after the last iteration, by one solve launch). A failed
+ * factorisation sets `stopped`, after which the remaining iterations change
+ * nothing. The state a loop leaves is bit-identical to that of the same
+ * number of stb_icp_correspond + stb_icp_update pairs. Which of the two
+ * correspondence maps an iteration uses is kept in the state, so a captured
+ * graph of any number of iterations replays correctly. */
 int stb_icp_iterate(const stb_icp_level* level, const stb_icp_params* params,
                     void* workspace, size_t workspace_bytes, int32_t num_iters,
                     void* stream);
