@@ -193,7 +193,8 @@ __device__ inline void term_adjoint(const float J[6], float w, float s, float r,
 template <bool GEOM, bool FEAT>
 __device__ inline void adjoint_pixel(const LevelView<float>& L, const float* m,
                                      const unsigned long long* __restrict__ map,
-                                     int64_t idx, const TermWeights& tw,
+                                     uint32_t tag, int64_t idx,
+                                     const TermWeights& tw,
                                      const float* lam, const float* x,
                                      const stb_icp_grads& G,
                                      double (&tacc)[kPose]) {
@@ -205,7 +206,7 @@ __device__ inline void adjoint_pixel(const LevelView<float>& L, const float* m,
   int ui, vi;
   if (!round_in_range(u, L.W, ui) || !round_in_range(v, L.H, vi)) return;
   const int64_t pos = int64_t(vi) * L.W + ui;
-  if (uint32_t(map[pos]) != uint32_t(idx)) return;
+  if (!key_owned(map[pos], uint32_t(idx), tag)) return;
   V3<float> pb{0.0f, 0.0f, 0.0f};  // dL/dp'
   if (GEOM) {
     const V3<float> q = ld3f(L.tp, pos);
@@ -332,6 +333,7 @@ __global__ void __launch_bounds__(kBlock)
     stb_icp_state* st = states + p;
     const LevelView<float> L = view_of<float>(pv.levels[p]);
     const unsigned long long* map = pair_map(st, maps0, maps1, pv.map_off[p], 0);
+    const uint32_t tag = uint32_t(st->map_epoch);
     const stb_icp_grads G = grads[p];
     float lam[6], x[6], m[12];
 #pragma unroll
@@ -348,7 +350,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
       const int64_t idx = base + k;
       if (idx < L.n)
-        adjoint_pixel<GEOM, FEAT>(L, m, map, idx, tw, lam, x, G, tacc);
+        adjoint_pixel<GEOM, FEAT>(L, m, map, tag, idx, tw, lam, x, G, tacc);
     }
     pose_partial(tacc, partials + int64_t(t) * kSlots, sh);
   }

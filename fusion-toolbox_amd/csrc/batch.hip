@@ -39,10 +39,11 @@ __global__ void __launch_bounds__(kBlock)
     for (int i = 0; i < 9; ++i) nm[i] = st->nm[i];
     unsigned long long* map =
         pair_map(st, maps0, maps1, pv.map_off[p], fixed_map);
+    const uint32_t tag = fixed_map ? kFreshTag : uint32_t(st->map_epoch);
     const int64_t base = int64_t(t - pv.tile_start[p]) * pv.tile_px;
     for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
       const int64_t idx = base + k;
-      if (idx < L.n) corresp_pixel(L, R, m, nm, idx, map, dthr2, cstar);
+      if (idx < L.n) corresp_pixel(L, R, m, nm, idx, map, tag, dthr2, cstar);
     }
   }
 }
@@ -65,13 +66,14 @@ __global__ void __launch_bounds__(kBlock)
     const LevelRes R = res_of(L);
     const int64_t off = pv.map_off[p];
     const unsigned long long* map = pair_map(st, maps0, maps1, off, fixed_map);
+    const uint32_t tag = fixed_map ? kFreshTag : uint32_t(st->map_epoch);
     const int first = pv.tile_start[p];
     const int ntiles = pv.tile_start[p + 1] - first;
     const int64_t base = int64_t(t - first) * pv.tile_px;
-    if (!fixed_map) {
-      // Clear this tile's share of the pair's other map for the next
-      // iteration; the pair's last tile also clears target pixels beyond
-      // the source count.
+    if (!fixed_map && tag == 0) {
+      // The pair's last tag on this map: clear this tile's share of the
+      // other map, which the next iteration uses with a fresh tag; the
+      // pair's last tile also clears target pixels beyond the source count.
       unsigned long long* next = pair_map(st, maps0, maps1, off, 1);
       int64_t end = base + pv.tile_px;
       if (t - first == ntiles - 1 && end < L.HW) end = L.HW;
@@ -87,7 +89,8 @@ __global__ void __launch_bounds__(kBlock)
     for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
     for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
       const int64_t idx = base + k;
-      if (idx < L.n) jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, idx, tw, acc);
+      if (idx < L.n)
+        jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, tw, acc);
     }
     block_partial(acc, partials + int64_t(t) * kSlots, sh);
   }
@@ -137,7 +140,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
     for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
       const int64_t idx = base + k;
-      if (idx < L.n) information_pixel(L, m, map, idx, acc);
+      if (idx < L.n) information_pixel(L, m, map, kFreshTag, idx, acc);
     }
     block_partial(acc, partials + int64_t(t) * kSlots, sh);
   }
@@ -328,6 +331,9 @@ extern "C" int stb_icp_batch_plan(int32_t num_pairs,
                 "stb_icp_batch_plan: pair %d has a NULL array", p);
     STB_REQUIRE(L.n_src > 0 && L.height > 0 && L.width > 0,
                 "stb_icp_batch_plan: pair %d is empty", p);
+    STB_REQUIRE(L.n_src <= kMaxKeyIndex,
+                "stb_icp_batch_plan: pair %d has more than %lld source pixels",
+                p, (long long)kMaxKeyIndex);
     STB_REQUIRE(L.storage == storage,
                 "stb_icp_batch_plan: all pairs must share one storage type");
     STB_REQUIRE(level_addressable(L),

@@ -35,6 +35,7 @@ __global__ void __launch_bounds__(kBlock)
   // flag and leaves the state alone), and not testing it first lets the
   // pose, the map choice and the pixel data load together.
   unsigned long long* __restrict__ map = pick_map(st, map0, map1, fixed_map);
+  const uint32_t tag = fixed_map ? kFreshTag : uint32_t(st->map_epoch);
   float m[12], nm[9];
 #pragma unroll
   for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
@@ -44,7 +45,7 @@ __global__ void __launch_bounds__(kBlock)
   const LevelRes R = res_of(L);
   for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < L.n;
        idx += int64_t(gridDim.x) * blockDim.x)
-    corresp_pixel(L, R, m, nm, idx, map, dthr2, cstar);
+    corresp_pixel(L, R, m, nm, idx, map, tag, dthr2, cstar);
 }
 
 // The correspondence launch of iteration k > 0 of a loop: every block first
@@ -84,7 +85,7 @@ __global__ void __launch_bounds__(kSolveBlock)
   const V3<float> ps = res_ld3<S>(R.sp, int(idx));
   const V3<float> ns = res_ld3<S>(R.sn, int(idx));
   const double row = st->transform[t < 12 ? t : 0];
-  // The flags stopped, iterations, ticket, map_sel (consecutive words) in
+  // The words stopped, iterations, map_epoch, map_sel (consecutive) in
   // one vector load, lane l of each wavefront taking word l % 4: a uniform
   // scalar load would be issued where first used, after the factorisation.
   const int32_t flag = (&st->stopped)[t & 3];
@@ -97,6 +98,7 @@ __global__ void __launch_bounds__(kSolveBlock)
   }
   const int stopped = __shfl(flag, 0, 64);
   const int iters = __shfl(flag, 1, 64);
+  const uint32_t tag = uint32_t(__shfl(flag, 2, 64));
   const int sel = __shfl(flag, 3, 64);
   __syncthreads();
   const bool ok = !stopped && s_ok != 0;
@@ -111,7 +113,10 @@ __global__ void __launch_bounds__(kSolveBlock)
 #pragma unroll
     for (int i = 0; i < 9; ++i) r.nm[i] = st->nm[i];
   }
-  const int use_sel = ok ? (sel ^ 1) : sel;
+  // This iteration's map use: the next after the last iteration's, unless the
+  // loop had already stopped (then nothing advances; the same keys are
+  // written again).
+  const MapUse use = stopped ? MapUse{tag, sel} : next_map_use(tag, sel);
   if (blockIdx.x == 0) {
     if (!stopped && t < kSlots) rec->tot[t] = tot[t];
     if (ok && t < 12) rec->T[t] = Tn[t];
@@ -122,15 +127,17 @@ __global__ void __launch_bounds__(kSolveBlock)
       for (int i = 0; i < 9; ++i) rec->nm[i] = r.nm[i];
       rec->active = !stopped;
       rec->ok = ok;
-      rec->map_sel = use_sel;
+      rec->map_sel = use.sel;
+      rec->map_epoch = int32_t(use.tag);
       rec->iterations = iters + 1;
     }
   }
-  unsigned long long* __restrict__ map = use_sel ? map1 : map0;
+  unsigned long long* __restrict__ map = use.sel ? map1 : map0;
   if (idx < hi)
-    corresp_pixel_at(L, R, r.m, r.nm, idx, vs, ps, ns, map, dthr2, cstar);
+    corresp_pixel_at(L, R, r.m, r.nm, idx, vs, ps, ns, map, use.tag, dthr2,
+                     cstar);
   for (int64_t i = idx + kSolveBlock; i < hi; i += kSolveBlock)
-    corresp_pixel(L, R, r.m, r.nm, i, map, dthr2, cstar);
+    corresp_pixel(L, R, r.m, r.nm, i, map, use.tag, dthr2, cstar);
 }
 
 // Per-pixel Jacobians of one iteration; every block leaves one FP64 partial
@@ -154,10 +161,9 @@ __global__ void __launch_bounds__(BLOCK)
   // ignores the partials of a stopped loop.
   const int sel = rec ? rec->map_sel : st->map_sel;
   const float* __restrict__ rt = rec ? rec->rt : st->rt;
-  unsigned long long* map = (sel ^ fixed_map) ? map1 : map0;
-  // A loop's map is emptied by its owners as they read it; one-off passes
-  // (fixed_map) leave theirs for repeated calls.
-  const bool clear = !fixed_map;
+  const uint32_t tag =
+      fixed_map ? kFreshTag : uint32_t(rec ? rec->map_epoch : st->map_epoch);
+  const unsigned long long* __restrict__ map = (sel ^ fixed_map) ? map1 : map0;
   const int64_t gtid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   float m[12];
 #pragma unroll
@@ -169,7 +175,7 @@ __global__ void __launch_bounds__(BLOCK)
   for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
   if (ONE_PX) {
     if (gtid < L.n)
-      jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, gtid, tw, acc, clear);
+      jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, tag, gtid, tw, acc);
   } else {
     // Each block takes a contiguous run of pixels, so the pixels beyond one
     // per thread spread over every block instead of doubling the work of the
@@ -179,12 +185,11 @@ __global__ void __launch_bounds__(BLOCK)
     const int64_t lo = int64_t(blockIdx.x) * chunk;
     const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
     for (int64_t idx = lo + threadIdx.x; idx < hi; idx += BLOCK)
-      jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, idx, tw, acc, clear);
+      jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, tw, acc);
   }
-  if (!fixed_map && rec == nullptr) {
-    // The first iteration of a loop (or a split update) also sweeps the
-    // other map, which a one-off pass may have left set: from here on both
-    // maps are empty at the start of every correspondence launch.
+  if (!fixed_map && tag == 0) {
+    // The last tag of this map: sweep the other one, which the next
+    // iteration uses with a fresh tag (one-off passes may have left it set).
     unsigned long long* __restrict__ next = sel ? map0 : map1;
     const int64_t chunk = (map_len + gridDim.x - 1) / gridDim.x;
     const int64_t lo = int64_t(blockIdx.x) * chunk;
@@ -229,7 +234,7 @@ __global__ void __launch_bounds__(kBlock)
   for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
   for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < L.n;
        idx += int64_t(gridDim.x) * blockDim.x)
-    information_pixel(L, m, map, idx, acc);
+    information_pixel(L, m, map, kFreshTag, idx, acc);
   block_partial(acc, partials + int64_t(blockIdx.x) * kSlots, sh);
 }
 
@@ -349,8 +354,9 @@ int check_level(const stb_icp_level* L, bool need_feats) {
   STB_REQUIRE(L->src_points && L->src_normals && L->src_mask && L->tgt_points &&
                   L->tgt_normals && L->tgt_mask,
               "level has a NULL source/target array");
-  STB_REQUIRE(L->n_src > 0 && L->n_src < (int64_t(1) << 31),
-              "n_src must be in [1, 2^31), got %lld", (long long)L->n_src);
+  STB_REQUIRE(L->n_src > 0 && L->n_src <= kMaxKeyIndex,
+              "n_src must be in [1, %lld], got %lld", (long long)kMaxKeyIndex,
+              (long long)L->n_src);
   STB_REQUIRE(L->height > 0 && L->width > 0, "bad target size %dx%d", L->width,
               L->height);
   STB_REQUIRE(L->storage == STB_F32 || L->storage == STB_BF16 ||

@@ -189,7 +189,7 @@ __device__ inline void corresp_pixel_at(const LevelView<S>& L,
                                         uint8_t valid_src, V3<float> ps,
                                         V3<float> ns,
                                         unsigned long long* __restrict__ map,
-                                        float dthr2, float cstar) {
+                                        uint32_t tag, float dthr2, float cstar) {
 #pragma clang fp contract(off)
   const V3<float> p = rigid_point(m, ps);
   const V3<float> nn = rigid_normal(nm, ns);
@@ -208,7 +208,8 @@ __device__ inline void corresp_pixel_at(const LevelView<S>& L,
   const bool keep = (valid != 0) & !(sq > dthr2) &
                     !angle_rejects(gate_cos(nn, qn), cstar);
   if (!keep) return;
-  atomicMin(map + pos, (unsigned long long)make_key(sqrtf(sq), uint32_t(idx)));
+  atomicMin(map + pos,
+            (unsigned long long)make_key(sqrtf(sq), uint32_t(idx), tag));
 }
 
 template <typename S>
@@ -216,11 +217,11 @@ __device__ inline void corresp_pixel(const LevelView<S>& L, const LevelRes& R,
                                      const float* m, const float* nm,
                                      int64_t idx,
                                      unsigned long long* __restrict__ map,
-                                     float dthr2, float cstar) {
+                                     uint32_t tag, float dthr2, float cstar) {
   const uint8_t valid_src = res_u8(R.sm, int(idx));
   const V3<float> ps = res_ld3<S>(R.sp, int(idx));
   const V3<float> ns = res_ld3<S>(R.sn, int(idx));
-  corresp_pixel_at(L, R, m, nm, idx, valid_src, ps, ns, map, dthr2, cstar);
+  corresp_pixel_at(L, R, m, nm, idx, valid_src, ps, ns, map, tag, dthr2, cstar);
 }
 
 // Adds one weighted term J' = w J, r' (already scaled) to the JtJ/Jr slots.
@@ -333,20 +334,13 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
   return sq;
 }
 
-// With `clear` (a Gauss-Newton loop's own map) the owner of a map entry
-// resets it to the sentinel after reading it: every entry the
-// correspondence launch set has an owner that projects onto it again here,
-// so the map is empty again when the launch ends, without a sweep over it.
-// Other pixels projecting onto the entry read either the owner's index or
-// the sentinel, both "not mine". The store comes last, after every load of
-// the pixel has issued: the map is not a restrict pointer (it is written),
-// so no later load could move above the store.
 template <typename S, bool GEOM, bool FEAT, int NB = 4>
 __device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
                                       const float* m,
-                                      const unsigned long long* map,
-                                      int64_t idx, const TermWeights& tw,
-                                      float (&acc)[kSlots], bool clear = false) {
+                                      const unsigned long long* __restrict__ map,
+                                      uint32_t tag, int64_t idx,
+                                      const TermWeights& tw,
+                                      float (&acc)[kSlots]) {
 #pragma clang fp contract(off)
   const uint8_t valid_src = res_u8(R.sm, int(idx));
   const V3<float> ps = res_ld3<S>(R.sp, int(idx));
@@ -358,12 +352,10 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
   const bool in_v = round_in_range(v, L.H, vi);
   if (!valid_src || !in_u || !in_v) return;
   const int pos = vi * L.W + ui;
-  // Only the owner half of the key (the low word: idx) is read. Loading the
-  // whole key would leave its unused high half's register free for reuse,
-  // which makes the compiler wait for the load before anything else issues.
   const __amdgpu_buffer_rsrc_t rmap = make_res(map, L.HW);
-  const uint32_t owner = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos, 0, 0);
-  const bool own = owner == uint32_t(idx);
+  const uint32_t klo = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos, 0, 0);
+  const uint32_t khi = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos + 4, 0, 0);
+  const bool own = key_owned((uint64_t(khi) << 32) | klo, uint32_t(idx), tag);
   if (GEOM) {
     const V3<float> q = res_ld3<S>(R.tp, pos);
     const V3<float> qn = res_ld3<S>(R.tn, pos);
@@ -387,7 +379,6 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
     acc[28] += use ? sq : 0.0f;
     acc[30] += use ? 1.0f : 0.0f;
   }
-  if (clear && own) const_cast<unsigned long long*>(map)[pos] = kKeySentinel;
 }
 
 // G^T G of source pixel idx from its matched target point
@@ -395,7 +386,8 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
 template <typename S>
 __device__ inline void information_pixel(const LevelView<S>& L, const float* m,
                                          const unsigned long long* __restrict__ map,
-                                         int64_t idx, float (&acc)[kSlots]) {
+                                         uint32_t tag, int64_t idx,
+                                         float (&acc)[kSlots]) {
 #pragma clang fp contract(off)
   if (!L.sm[idx]) return;
   const V3<float> p = rigid_point(m, ld3f(L.sp, idx));
@@ -404,7 +396,7 @@ __device__ inline void information_pixel(const LevelView<S>& L, const float* m,
   int ui, vi;
   if (!round_in_range(u, L.W, ui) || !round_in_range(v, L.H, vi)) return;
   const int64_t pos = int64_t(vi) * L.W + ui;
-  if (uint32_t(map[pos]) != uint32_t(idx)) return;
+  if (!key_owned(map[pos], uint32_t(idx), tag)) return;
   const V3<float> q = ld3f(L.tp, pos);
   const float r0[6] = {0.f, q.z, -q.y, 1.f, 0.f, 0.f};
   const float r1[6] = {-q.z, 0.f, q.x, 0.f, 1.f, 0.f};
@@ -659,12 +651,13 @@ __device__ inline void store_linearization(stb_icp_state* st,
 // arithmetic only. It runs in a launch of its own, so its register count
 // does not bound the occupancy of the per-pixel kernels. `w` is LDS of at
 // least 25 doubles whose first 12 hold rows 0..2 of the current pose;
-// thread 0 passes the state's iteration count and map_sel, read early by the
-// caller so that their latency hides behind the partial sums. Flips map_sel
-// on success.
+// thread 0 passes the state's iteration count, map_sel and map_epoch, read
+// early by the caller so that their latency hides behind the partial sums.
+// Advances the map use (next_map_use) whether or not the update succeeds:
+// the iteration wrote its map either way.
 __device__ inline void gn_finalize(stb_icp_state* st, const double* tot,
                                    double gw, double fw, double* w, int iters,
-                                   int sel) {
+                                   int sel, uint32_t tag) {
   const int t = threadIdx.x;
   store_linearization(st, tot);
   __syncthreads();
@@ -738,10 +731,12 @@ __device__ inline void gn_finalize(stb_icp_state* st, const double* tot,
       st->best_residual = (fw * tot[28] + gw * tot[27]) / count;
       st->best_match_count = count;
       st->iterations = iters + 1;
-      st->map_sel = sel ^ 1;
     } else {
       st->stopped = 1;
     }
+    const MapUse nu = next_map_use(tag, sel);
+    st->map_epoch = int32_t(nu.tag);
+    st->map_sel = nu.sel;
     w[24] = ok ? 1.0 : 0.0;
   }
   __syncthreads();
@@ -776,15 +771,17 @@ __device__ inline void finish_partials(stb_icp_state* st, const double* partials
   // The state loads of the update issue before the partial loads.
   double row = 0.0;
   int iters = 0, sel = 0;
+  uint32_t tag = 0;
   if (update && t < 12) row = st->transform[t];
   if (update && t == 0) {
     iters = st->iterations;
     sel = st->map_sel;
+    tag = uint32_t(st->map_epoch);
   }
   sum_partials<BLOCK>(partials, count, sh, tot);
   if (update) {
     if (t < 12) sh[t] = row;
-    gn_finalize(st, tot, gw, fw, sh, iters, sel);
+    gn_finalize(st, tot, gw, fw, sh, iters, sel, tag);
   } else if (mode == kStoreInformation) {
     store_information(st, tot);
   } else {
@@ -812,7 +809,9 @@ struct SolveRecord {
   int32_t active;      // 0: the loop had already stopped, nothing to commit
   int32_t ok;          // every Cholesky pivot was positive
   int32_t map_sel;     // correspondence map this iteration uses
+  int32_t map_epoch;   // key tag this iteration writes
   int32_t iterations;  // state.iterations after a successful update
+  int32_t pad;
 };
 
 __device__ inline double readlane_f64(double v, int lane) {
@@ -889,7 +888,7 @@ __device__ inline bool wave_solve_update(const double* tot, const double* To,
 }
 
 // What gn_finalize stores, from a SolveRecord (every thread of a block of at
-// least 102 threads calls it). Each thread loads the values it stores first,
+// least 103 threads calls it). Each thread loads the values it stores first,
 // so the block pays one memory round trip, not one per field.
 __device__ inline void commit_record(stb_icp_state* __restrict__ st,
                                      const SolveRecord* __restrict__ rec,
@@ -917,9 +916,15 @@ __device__ inline void commit_record(stb_icp_state* __restrict__ st,
     v1 = c;
   } else if (t == 101) {
     i0 = rec->iterations;
-    i1 = rec->map_sel;
+  } else if (t == 102) {
+    i0 = rec->map_sel;
+    i1 = rec->map_epoch;
   }
   if (!active) return;
+  if (t == 102) {  // the map use advances whether or not the update succeeded
+    st->map_sel = i0;
+    st->map_epoch = i1;
+  }
   if (t < 36)
     st->last_jtj[t] = v0;
   else if (t < 42)
@@ -949,7 +954,6 @@ __device__ inline void commit_record(stb_icp_state* __restrict__ st,
     st->best_match_count = v1;
   } else if (t == 101) {
     st->iterations = i0;
-    st->map_sel = i1;
   }
 }
 
@@ -982,7 +986,7 @@ __device__ inline void reset_state(stb_icp_state* st,
   st->last_geom_count = st->last_feat_count = 0.0;
   st->stopped = 0;
   st->iterations = 0;
-  st->ticket = 0;
+  st->map_epoch = kFreshTag;
   st->map_sel = 0;
   set_pose_floats(st);
 }

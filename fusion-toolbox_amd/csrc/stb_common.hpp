@@ -196,16 +196,42 @@ __device__ inline float ldw(const __half* p, int64_t i) { return ldf(p, i); }
 // Correspondence keys.
 //
 // Reference layout (kept by the drop-in entry point): (index << 32) | bits(d).
-// Internal layout of the fused path: (bits(d) << 32) | index, so one 64-bit
-// unsigned atomic min yields the lowest distance and, among equal distances,
-// the lowest source index -- the order of the sequential CPU path. Distances
-// are non-negative floats, whose bit patterns order like their values.
+// Internal layout of the fused path, from the top bit down:
+//   tag (8 bits) | bits(d) (31 bits) | index (25 bits),
+// so one 64-bit unsigned atomic min yields, among keys of one tag, the lowest
+// distance and among equal distances the lowest source index -- the order of
+// the sequential CPU path (distances are non-negative floats, whose bit
+// patterns order like their values). A Gauss-Newton loop gives each
+// iteration a tag one lower than the last: every key it writes is below
+// every key an earlier iteration left, and an owner must match the tag too,
+// so the map needs no clearing between iterations (only when the tag runs
+// out, every 256 iterations). Source indices must stay below 2^25 - 1.
 // ---------------------------------------------------------------------------
 constexpr uint64_t kRefSentinel = 0x0fffffff7f800000ull;
-constexpr uint64_t kKeySentinel = 0x7f8000000fffffffull;
+constexpr uint64_t kKeySentinel = ~0ull;  // above every key
+constexpr int kKeyIdxBits = 25;
+constexpr uint64_t kKeyIdxMask = (uint64_t(1) << kKeyIdxBits) - 1;
+constexpr int64_t kMaxKeyIndex = int64_t(kKeyIdxMask) - 1;
+constexpr uint32_t kFreshTag = 255;  // the tag of a freshly cleared map
 
-__device__ inline uint64_t make_key(float dist, uint32_t idx) {
-  return (uint64_t(__float_as_uint(dist)) << 32) | uint64_t(idx);
+__device__ inline uint64_t make_key(float dist, uint32_t idx, uint32_t tag) {
+  return (uint64_t(tag) << 56) |
+         (uint64_t(__float_as_uint(dist)) << kKeyIdxBits) | uint64_t(idx);
+}
+
+__device__ inline bool key_owned(uint64_t key, uint32_t idx, uint32_t tag) {
+  return (key & kKeyIdxMask) == idx && uint32_t(key >> 56) == tag;
+}
+
+// The tag and map of the iteration after one that used (tag, sel): one tag
+// lower on the same map, or, after tag 0, a fresh tag on the other map,
+// which that iteration's Jacobian launch swept.
+struct MapUse {
+  uint32_t tag;
+  int sel;
+};
+__host__ __device__ inline MapUse next_map_use(uint32_t tag, int sel) {
+  return tag == 0 ? MapUse{kFreshTag, sel ^ 1} : MapUse{tag - 1, sel};
 }
 
 constexpr int kBlock = 256;

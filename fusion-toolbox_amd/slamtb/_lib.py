@@ -63,7 +63,7 @@ class StbIcpState(ctypes.Structure):
         ("last_geom_r2", _f64), ("last_feat_r2", _f64),
         ("last_geom_count", _f64), ("last_feat_count", _f64),
         ("rt", _f32 * 12), ("nm", _f32 * 9), ("stopped", _i32),
-        ("iterations", _i32), ("ticket", _i32), ("map_sel", _i32),
+        ("iterations", _i32), ("map_epoch", _i32), ("map_sel", _i32),
     ]
 
 

@@ -377,7 +377,7 @@ typedef struct stb_icp_state {
   float nm[9];            /* inverse-transpose of rt's rotation */
   int32_t stopped;        /* 1 after a failed Cholesky (the loop's `break`) */
   int32_t iterations;     /* accepted iterations */
-  int32_t ticket;         /* reserved, zero */
+  int32_t map_epoch;      /* key tag the next iteration writes (255..0) */
   int32_t map_sel;        /* correspondence map the next iteration uses */
 } stb_icp_state;
 

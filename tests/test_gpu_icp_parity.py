@@ -263,7 +263,7 @@ def state_bits(st):
                 residual=st.best_residual, count=st.best_match_count,
                 r2=(st.last_geom_r2, st.last_feat_r2), rt=tuple(st.rt),
                 nm=tuple(st.nm), iterations=st.iterations, stopped=st.stopped,
-                map_sel=st.map_sel)
+                map_sel=st.map_sel, map_epoch=st.map_epoch)
 
 
 def midstop_pair():
@@ -332,40 +332,46 @@ def test_loop_equals_split_iterations(case):
         assert loop["iterations"] == iters and loop["stopped"] == 0
 
 
-def workspace_maps(run, level):
-    """The two correspondence maps inside the fused workspace (state, then
-    each map padded to 256 bytes), as uint64 tensors."""
-    from slamtb.registration._fused import STATE_BYTES
-    pad = lambda b: (b + 255) // 256 * 256
-    hw = level.height * level.width
-    off0 = pad(STATE_BYTES)
-    off1 = off0 + pad(hw * 8)
-    ws = run.workspace
-    return [ws[o:o + hw * 8].view(torch.int64) for o in (off0, off1)]
-
-
-def test_loop_leaves_both_maps_empty():
-    """A loop's Jacobian launches empty the correspondence map as its owners
-    read it (no sweep), and the loop's first iteration sweeps the map a
-    one-off pass left set: after any loop both maps hold only the sentinel,
-    so the next correspondence pass starts from an empty map."""
-    sentinel = 0x7f8000000fffffff
-    P = prepared_pair(7, 640, 480, channels=3)
+def test_map_tags_wrap_around():
+    """A loop tags each iteration's correspondence keys one lower than the
+    last (255, 254, ...) instead of clearing the map; after tag 0 it sweeps
+    the other map and continues there with 255. Across two wraps, with a
+    one-off linearisation (which leaves the spare map set) in between, the
+    loop and the split entry points stay bit-identical and the pose matches
+    the oracle's 600-iteration loop."""
+    k = synthetic.intrinsics(160, 120).astype(np.float32)
+    P = prepared_pair(3, 160, 120, kcam=k)
+    T0 = small_motion(7, 2.0, 0.02)
     g = to_dev(P, DEV)
     level = Level(P["kcam"], g["src_pts"], g["src_nrm"], g["src_mask"],
-                  g["tgt_pts"], g["tgt_nrm"], g["tgt_mask"], g["src_feats"],
-                  g["tgt_feats"])
-    params = make_params(10.0, 1.0, 0.1, ANG20, 2.75, True)
+                  g["tgt_pts"], g["tgt_nrm"], g["tgt_mask"])
+    params = make_params(1.0, 0.0, 0.1, ANG20, 2.75, False)
     run = runner_for(DEV)
-    run.begin(level, torch.from_numpy(small_motion(5)))
-    run.iterate(level, params, 4)
-    for m in workspace_maps(run, level):
-        assert bool((m == sentinel).all())
-    run.linearize(level, params)  # sets the spare map
+    run.begin(level, torch.from_numpy(T0))
+    run.iterate(level, params, 10)
     st = run.read_state()
-    assert max(st.last_geom_count, st.last_feat_count) > 10000
-    assert any(not bool((m == sentinel).all()) for m in workspace_maps(run, level))
-    run.iterate(level, params, 3)
-    for m in workspace_maps(run, level):
-        assert bool((m == sentinel).all())
-    assert run.read_state().iterations == 7
+    assert (st.map_epoch, st.map_sel) == (245, 0)
+    run.iterate(level, params, 290)
+    run.linearize(level, params)
+    run.iterate(level, params, 300)
+    loop = state_bits(run.read_state())
+    assert (loop["map_epoch"], loop["map_sel"]) == (255 - (600 - 512), 0)
+    run.begin(level, torch.from_numpy(T0))
+    for i in range(600):
+        if i == 300:
+            run.linearize(level, params)
+        run.correspond(level, params)
+        run.update(level, params)
+    split = state_bits(run.read_state())
+    for key in loop:
+        assert loop[key] == split[key], key
+    assert loop["iterations"] == 600 and loop["stopped"] == 0
+    ref = orc.icp_estimate(P["kcam"], P["src_pts"], P["src_nrm"], P["src_mask"],
+                           P["tgt_pts"], P["tgt_nrm"], P["tgt_mask"], 600,
+                           geom_weight=1.0, feat_weight=0.0,
+                           distance_threshold=0.1, normals_angle_thresh=ANG20,
+                           transform=T0)
+    T = np.array(loop["transform"]).reshape(4, 4)
+    err = orc.pose_error(T, ref["transform"])
+    assert ref["iters"] == 600
+    assert err[0] < 1e-6 and err[1] < 1e-6, err
