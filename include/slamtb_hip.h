@@ -424,11 +424,11 @@ int stb_icp_update(const stb_icp_level* level, const stb_icp_params* params,
                    void* workspace, size_t workspace_bytes, void* stream);
 
 /* The Jacobian launch of one iteration alone, over the correspondence map
- * of the last stb_icp_linearize call (an iteration's own map is emptied by
- * its Jacobian launch as it is read): per-block partials into the
- * workspace, with no reduction, no pose update and both maps untouched, so
- * that repeated calls do identical work. For timing the dominant launch of
- * an iteration on its own. */
+ * of the last stb_icp_linearize call (a loop's own map holds tagged keys
+ * that only its iterations read): per-block partials into the workspace,
+ * with no reduction, no pose update and both maps untouched, so that
+ * repeated calls do identical work. For timing the dominant launch of an
+ * iteration on its own. */
 int stb_icp_jacobians(const stb_icp_level* level, const stb_icp_params* params,
                       void* workspace, size_t workspace_bytes, void* stream);
 
