@@ -142,16 +142,22 @@ struct GridEntry<double> {
 // The same top-K as knn_kernel, over the 27 grid cells around the query
 // (cells are at least the distance bound wide, so they hold every target
 // within it). Candidates arrive out of index order, so ties compare
-// (distance, index) and the result equals the brute-force search's.
+// (distance, index) and the result equals the brute-force search's. Thread
+// t takes query order[t] (order NULL: query t): with the queries ordered by
+// grid cell, the lanes of a wavefront scan the same cells, so their
+// candidate loads coalesce into the same lines and their loops run the same
+// trip counts.
 template <typename T, int K>
 __global__ void __launch_bounds__(kBlock)
     knn_grid_kernel(const T* __restrict__ entries, const int64_t* __restrict__ start,
                     int nx, int ny, int nz, T lox, T loy, T loz, T inv, int64_t m,
-                    const T* __restrict__ qry, int64_t n, int k, T bound2,
+                    const T* __restrict__ qry, int64_t n,
+                    const int64_t* __restrict__ order, int k, T bound2,
                     T* __restrict__ out_d, int64_t* __restrict__ out_i) {
 #pragma clang fp contract(off)
-  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q >= n) return;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t q = order ? order[t] : t;
   const V3<T> p = ld3(qry, q);
   const T inf = T(INFINITY);
   T d[K];
@@ -406,22 +412,24 @@ struct GridArgs {
 
 template <typename T, int K>
 void launch_knn_grid(const GridArgs& g, int64_t m, const void* qry, int64_t n,
-                     int k, double bound, void* d, int64_t* i, hipStream_t s) {
+                     const int64_t* order, int k, double bound, void* d,
+                     int64_t* i, hipStream_t s) {
   hipLaunchKernelGGL((knn_grid_kernel<T, K>), dim3(grid_for(n)), dim3(kBlock), 0,
                      s, static_cast<const T*>(g.entries), g.start, g.nx,
                      g.ny, g.nz, T(g.lo[0]), T(g.lo[1]), T(g.lo[2]),
-                     T(1) / T(g.cell), m, static_cast<const T*>(qry), n, k,
-                     T(bound) * T(bound), static_cast<T*>(d), i);
+                     T(1) / T(g.cell), m, static_cast<const T*>(qry), n, order,
+                     k, T(bound) * T(bound), static_cast<T*>(d), i);
 }
 
 template <typename T>
 void knn_grid_dispatch(const GridArgs& g, int64_t m, const void* qry, int64_t n,
-                       int k, double bound, void* d, int64_t* i, hipStream_t s) {
-  if (k <= 1) launch_knn_grid<T, 1>(g, m, qry, n, k, bound, d, i, s);
-  else if (k <= 2) launch_knn_grid<T, 2>(g, m, qry, n, k, bound, d, i, s);
-  else if (k <= 4) launch_knn_grid<T, 4>(g, m, qry, n, k, bound, d, i, s);
-  else if (k <= 8) launch_knn_grid<T, 8>(g, m, qry, n, k, bound, d, i, s);
-  else launch_knn_grid<T, 16>(g, m, qry, n, k, bound, d, i, s);
+                       const int64_t* order, int k, double bound, void* d,
+                       int64_t* i, hipStream_t s) {
+  if (k <= 1) launch_knn_grid<T, 1>(g, m, qry, n, order, k, bound, d, i, s);
+  else if (k <= 2) launch_knn_grid<T, 2>(g, m, qry, n, order, k, bound, d, i, s);
+  else if (k <= 4) launch_knn_grid<T, 4>(g, m, qry, n, order, k, bound, d, i, s);
+  else if (k <= 8) launch_knn_grid<T, 8>(g, m, qry, n, order, k, bound, d, i, s);
+  else launch_knn_grid<T, 16>(g, m, qry, n, order, k, bound, d, i, s);
 }
 
 template <typename T>
@@ -492,7 +500,8 @@ extern "C" int stb_knn_grid(int32_t dtype, const void* entries,
                             const int64_t* cell_start,
                             int32_t nx, int32_t ny, int32_t nz, double lo_x,
                             double lo_y, double lo_z, double cell, int64_t m,
-                            const void* query, int64_t n, int32_t k,
+                            const void* query, int64_t n,
+                            const int64_t* query_order, int32_t k,
                             double distance_upper_bound, void* out_sq_dist,
                             int64_t* out_index, void* stream) {
   STB_REQUIRE(dtype == STB_F32 || dtype == STB_F64,
@@ -511,7 +520,7 @@ extern "C" int stb_knn_grid(int32_t dtype, const void* entries,
   STB_REQUIRE(dtype == STB_F64 || m < (int64_t(1) << 31),
               "knn_grid: float32 entries hold 31-bit indices");
   GridArgs g{entries, cell_start, nx, ny, nz, {lo_x, lo_y, lo_z}, cell};
-  STB_FLOAT_DISPATCH(dtype, knn_grid_dispatch, g, m, query, n, k,
+  STB_FLOAT_DISPATCH(dtype, knn_grid_dispatch, g, m, query, n, query_order, k,
                      distance_upper_bound, out_sq_dist, out_index,
                      as_stream(stream));
   return check_launch("knn_grid");

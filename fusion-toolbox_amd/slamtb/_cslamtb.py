@@ -626,6 +626,8 @@ class KnnGrid:
                                        device=t.device)).to(torch.int64).contiguous()
 
     def query(self, query, k, distance_upper_bound):
+        """Threads take the queries in grid-cell order (a stable sort of
+        their cells), so that a wavefront's queries scan the same cells."""
         what = "knn"
         q = _in(query, self.dtype).view(-1, 3)
         n = q.size(0)
@@ -633,11 +635,16 @@ class KnnGrid:
         i = torch.empty(n, k, dtype=torch.int64, device=q.device)
         nx, ny, nz = self.dims
         with torch.cuda.device(q.device):
+            cells = torch.empty(n, dtype=torch.int32, device=q.device)
+            check(lib().stb_grid_cells(dtype_code(q), ptr(q), n, self.lo[0],
+                                       self.lo[1], self.lo[2], self.cell, nx, ny,
+                                       nz, ptr(cells), stream()), what)
+            order = torch.sort(cells, stable=True)[1].contiguous()
             check(lib().stb_knn_grid(
                 dtype_code(q), ptr(self.entries), ptr(self.start),
                 nx, ny, nz, self.lo[0], self.lo[1], self.lo[2], self.cell, self.m,
-                ptr(q), n, int(k), float(distance_upper_bound), ptr(d), ptr(i),
-                stream()), what)
+                ptr(q), n, ptr(order), int(k), float(distance_upper_bound), ptr(d),
+                ptr(i), stream()), what)
         return d, i
 
 

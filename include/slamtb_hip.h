@@ -252,12 +252,16 @@ int stb_grid_cells(int32_t dtype, const void* points, int64_t m, double lo_x,
  * entries [m,4] holds the targets stably sorted by cell as (x, y, z, index),
  * the index's int32 bits in the fourth float for STB_F32 and its value for
  * STB_F64; cell_start [nx*ny*nz + 1] the first sorted position of each cell.
- * Same result as stb_knn (ties broken by original index). */
+ * query_order (device [n] permutation of the queries, or NULL) is the order
+ * in which threads take the queries: ordered by grid cell (stb_grid_cells
+ * of the queries, stably sorted), a wavefront's queries scan the same cells
+ * and its candidate loads coalesce. Each query's row of the outputs is its
+ * own either way. Same result as stb_knn (ties broken by original index). */
 int stb_knn_grid(int32_t dtype, const void* entries, const int64_t* cell_start,
                  int32_t nx, int32_t ny, int32_t nz,
                  double lo_x, double lo_y, double lo_z, double cell, int64_t m,
-                 const void* query, int64_t n, int32_t k,
-                 double distance_upper_bound, void* out_sq_dist,
+                 const void* query, int64_t n, const int64_t* query_order,
+                 int32_t k, double distance_upper_bound, void* out_sq_dist,
                  int64_t* out_index, void* stream);
 
 /* NearestNeighborsOp.forward (src/nearest_neighbors.cpp:7,
