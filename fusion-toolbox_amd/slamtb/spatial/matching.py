@@ -28,6 +28,17 @@ class FPCLMatcherOp(torch.autograd.Function):
             self.kcam = kcam
             self.distance_threshold = distance_threshold
             self.normals_angle_thresh = normals_angle_thresh
+            self._kcam_dev = {}
+
+        def kcam_on(self, device, dtype):
+            """The intrinsic matrix on `device` in `dtype`, copied once: a
+            forward/backward pass issues no host-to-device copy, so it can be
+            captured in a hipGraph."""
+            key = (str(device), dtype)
+            if key not in self._kcam_dev:
+                self._kcam_dev[key] = torch.as_tensor(
+                    self.kcam.matrix).to(device=device, dtype=dtype)
+            return self._kcam_dev[key]
 
     class Match:
         def __init__(self):
@@ -44,7 +55,7 @@ class FPCLMatcherOp(torch.autograd.Function):
         device = source_points.device
         dtype = source_points.dtype
         size = source_points.size(0)
-        kcam = target.kcam.matrix.to(device)
+        kcam = target.kcam_on(device, target.kcam.matrix.dtype)
 
         # The source points arrive already transformed: identity pose and a
         # one-element mask (every source point takes part, matching.py:44-49).
@@ -86,7 +97,8 @@ class FPCLMatcherOp(torch.autograd.Function):
                                 dtype=dl_features.dtype)
         _FPCLMatcherOp.backward(ctx.target.features, ctx.source_points,
                                 ctx.match_mask, dl_features.contiguous(),
-                                ctx.target.kcam.matrix.to(dl_features.device),
+                                ctx.target.kcam_on(dl_features.device,
+                                                   ctx.target.kcam.matrix.dtype),
                                 ctx.grad_precision, dx_points)
         return dx_points, None, None, None, None
 
