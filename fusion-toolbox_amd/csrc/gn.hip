@@ -181,10 +181,24 @@ __global__ void __launch_bounds__(BLOCK)
     // per thread spread over every block instead of doubling the work of the
     // first ones (with one 1024-thread block per CU, a grid-stride loop left
     // 44 CUs with twice the work at 640x480).
+    // The first two pixels' source loads issue together, so a thread's
+    // second pixel does not wait a round trip behind its first (past the end
+    // of the arrays the buffer loads return 0).
     const int64_t chunk = (L.n + gridDim.x - 1) / gridDim.x;
     const int64_t lo = int64_t(blockIdx.x) * chunk;
     const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
-    for (int64_t idx = lo + threadIdx.x; idx < hi; idx += BLOCK)
+    const int64_t i0 = lo + threadIdx.x, i1 = i0 + BLOCK;
+    const uint8_t vs0 = res_u8(R.sm, int(i0));
+    const V3<float> ps0 = res_ld3<S>(R.sp, int(i0));
+    const uint8_t vs1 = res_u8(R.sm, int(i1));
+    const V3<float> ps1 = res_ld3<S>(R.sp, int(i1));
+    if (i0 < hi)
+      jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, i0, vs0, ps0, tw,
+                                           acc);
+    if (i1 < hi)
+      jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, i1, vs1, ps1, tw,
+                                           acc);
+    for (int64_t idx = i1 + BLOCK; idx < hi; idx += BLOCK)
       jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, tw, acc);
   }
   if (!fixed_map && tag == 0) {

@@ -335,15 +335,14 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
 }
 
 template <typename S, bool GEOM, bool FEAT, int NB = 4>
-__device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
-                                      const float* m,
-                                      const unsigned long long* __restrict__ map,
-                                      uint32_t tag, int64_t idx,
-                                      const TermWeights& tw,
-                                      float (&acc)[kSlots]) {
+__device__ inline void jacobian_pixel_at(const LevelView<S>& L,
+                                         const LevelRes& R, const float* m,
+                                         const unsigned long long* __restrict__ map,
+                                         uint32_t tag, int64_t idx,
+                                         uint8_t valid_src, V3<float> ps,
+                                         const TermWeights& tw,
+                                         float (&acc)[kSlots]) {
 #pragma clang fp contract(off)
-  const uint8_t valid_src = res_u8(R.sm, int(idx));
-  const V3<float> ps = res_ld3<S>(R.sp, int(idx));
   const V3<float> p = rigid_point(m, ps);
   float u, v;
   project(L.fx, L.fy, L.cx, L.cy, p, u, v);
@@ -379,6 +378,20 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
     acc[28] += use ? sq : 0.0f;
     acc[30] += use ? 1.0f : 0.0f;
   }
+}
+
+// jacobian_pixel_at with the source pixel's mask and point loaded here.
+template <typename S, bool GEOM, bool FEAT, int NB = 4>
+__device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
+                                      const float* m,
+                                      const unsigned long long* __restrict__ map,
+                                      uint32_t tag, int64_t idx,
+                                      const TermWeights& tw,
+                                      float (&acc)[kSlots]) {
+  const uint8_t valid_src = res_u8(R.sm, int(idx));
+  const V3<float> ps = res_ld3<S>(R.sp, int(idx));
+  jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, valid_src, ps,
+                                       tw, acc);
 }
 
 // G^T G of source pixel idx from its matched target point
