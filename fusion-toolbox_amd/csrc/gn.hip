@@ -183,15 +183,18 @@ __global__ void __launch_bounds__(BLOCK)
     // 44 CUs with twice the work at 640x480).
     // The first two pixels' source loads issue together, so a thread's
     // second pixel does not wait a round trip behind its first (past the end
-    // of the arrays the buffer loads return 0).
+    // of the arrays the buffer loads return 0). A thread without a second
+    // pixel re-reads its first: the next run belongs to a block on another
+    // XCD, whose lines would cross the fabric a second time.
     const int64_t chunk = (L.n + gridDim.x - 1) / gridDim.x;
     const int64_t lo = int64_t(blockIdx.x) * chunk;
     const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
     const int64_t i0 = lo + threadIdx.x, i1 = i0 + BLOCK;
+    const int64_t j1 = i1 < hi ? i1 : i0;
     const uint8_t vs0 = res_u8(R.sm, int(i0));
     const V3<float> ps0 = res_ld3<S>(R.sp, int(i0));
-    const uint8_t vs1 = res_u8(R.sm, int(i1));
-    const V3<float> ps1 = res_ld3<S>(R.sp, int(i1));
+    const uint8_t vs1 = res_u8(R.sm, int(j1));
+    const V3<float> ps1 = res_ld3<S>(R.sp, int(j1));
     if (i0 < hi)
       jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, i0, vs0, ps0, tw,
                                            acc);
