@@ -56,8 +56,10 @@ __global__ void __launch_bounds__(kBlock)
 // Every load that does not depend on the new pose -- the thread's first
 // source pixel, the previous pose, the partials, the state flags -- issues
 // before the first wait, so the update costs one memory round trip plus the
-// factorisation; barriers do not wait for outstanding global loads.
-template <typename S>
+// factorisation; barriers do not wait for outstanding global loads. With
+// TWO_PX (runs longer than a block) the thread's second source pixel loads
+// up front too.
+template <typename S, bool TWO_PX>
 __global__ void __launch_bounds__(kSolveBlock)
     corresp_solve_kernel(stb_icp_level Ld, const stb_icp_state* __restrict__ st,
                          SolveRecord* __restrict__ rec,
@@ -84,6 +86,18 @@ __global__ void __launch_bounds__(kSolveBlock)
   const uint8_t vs = res_u8(R.sm, int(idx));
   const V3<float> ps = res_ld3<S>(R.sp, int(idx));
   const V3<float> ns = res_ld3<S>(R.sn, int(idx));
+  // The thread's second pixel, or its first again (see
+  // jacobian_fused_kernel). Only with TWO_PX: loads return in issue order,
+  // so needless ones would delay the partials.
+  const int64_t idx1 = idx + kSolveBlock;
+  uint8_t vs1 = 0;
+  V3<float> ps1 = {0.0f, 0.0f, 0.0f}, ns1 = {0.0f, 0.0f, 0.0f};
+  if (TWO_PX) {
+    const int64_t j1 = idx1 < hi ? idx1 : idx;
+    vs1 = res_u8(R.sm, int(j1));
+    ps1 = res_ld3<S>(R.sp, int(j1));
+    ns1 = res_ld3<S>(R.sn, int(j1));
+  }
   const double row = st->transform[t < 12 ? t : 0];
   // The words stopped, iterations, map_epoch, map_sel (consecutive) in
   // one vector load, lane l of each wavefront taking word l % 4: a uniform
@@ -136,7 +150,10 @@ __global__ void __launch_bounds__(kSolveBlock)
   if (idx < hi)
     corresp_pixel_at(L, R, r.m, r.nm, idx, vs, ps, ns, map, use.tag, dthr2,
                      cstar);
-  for (int64_t i = idx + kSolveBlock; i < hi; i += kSolveBlock)
+  if (TWO_PX && idx1 < hi)
+    corresp_pixel_at(L, R, r.m, r.nm, idx1, vs1, ps1, ns1, map, use.tag, dthr2,
+                     cstar);
+  for (int64_t i = TWO_PX ? idx1 + kSolveBlock : idx1; i < hi; i += kSolveBlock)
     corresp_pixel(L, R, r.m, r.nm, i, map, use.tag, dthr2, cstar);
 }
 
@@ -492,14 +509,20 @@ void launch_iterations(const stb_icp_level& L, const stb_icp_params& P,
   const float dthr2 = float(P.distance_thresh * P.distance_thresh);
   const float cstar = angle_gate_cos_f32(P.normal_angle_thresh);
   const int partials = jac_shape(P, L.n_src).grid;
+  const int cgrid = corresp_solve_grid(P, L.n_src);
+  const bool two_px = (L.n_src + cgrid - 1) / cgrid > kSolveBlock;
   for (int it = 0; it < count; ++it) {
     if (it == 0)
       hipLaunchKernelGGL(corresp_fused_kernel<S>,
                          dim3(corresp_grid(P, L.n_src)), dim3(kBlock), 0,
                          stream, L, w.st, w.map[0], w.map[1], dthr2, cstar, 0);
+    else if (two_px)
+      hipLaunchKernelGGL((corresp_solve_kernel<S, true>), dim3(cgrid),
+                         dim3(kSolveBlock), 0, stream, L, w.st, w.rec,
+                         w.partials, partials, w.map[0], w.map[1], dthr2,
+                         cstar);
     else
-      hipLaunchKernelGGL(corresp_solve_kernel<S>,
-                         dim3(corresp_solve_grid(P, L.n_src)),
+      hipLaunchKernelGGL((corresp_solve_kernel<S, false>), dim3(cgrid),
                          dim3(kSolveBlock), 0, stream, L, w.st, w.rec,
                          w.partials, partials, w.map[0], w.map[1], dthr2,
                          cstar);
