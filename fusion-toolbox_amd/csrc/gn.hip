@@ -80,6 +80,18 @@ __global__ void __launch_bounds__(kSolveBlock)
   const int64_t lo = int64_t(blockIdx.x) * chunk;
   const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
   const int64_t idx = lo + t;
+  // The partials first: the sum waits only for them, the loads below stay
+  // in flight through it (loads return in issue order).
+  double pv[kSumBatch];
+  load_partial_batch<kSolveBlock>(partials, count, t / kSlots, pv);
+  // Then what the solve uses (the previous pose row and the words stopped,
+  // iterations, map_epoch, map_sel, consecutive, in one vector load, lane l
+  // of each wavefront taking word l % 4: a uniform scalar load would be
+  // issued where first used, after the factorisation), then the source.
+  const double row = st->transform[t < 12 ? t : 0];
+  const int32_t flag = (&st->stopped)[t & 3];
+  // Keeps the scheduler from hoisting the source loads above these.
+  __builtin_amdgcn_sched_barrier(0);
   // Unconditional, so that no branch waits for them (past the end of the
   // arrays the buffer loads return 0).
   const LevelRes R = res_of(L);
@@ -98,12 +110,7 @@ __global__ void __launch_bounds__(kSolveBlock)
     ps1 = res_ld3<S>(R.sp, int(j1));
     ns1 = res_ld3<S>(R.sn, int(j1));
   }
-  const double row = st->transform[t < 12 ? t : 0];
-  // The words stopped, iterations, map_epoch, map_sel (consecutive) in
-  // one vector load, lane l of each wavefront taking word l % 4: a uniform
-  // scalar load would be issued where first used, after the factorisation.
-  const int32_t flag = (&st->stopped)[t & 3];
-  sum_partials<kSolveBlock>(partials, count, sh, tot);
+  sum_partials_from<kSolveBlock>(pv, partials, count, sh, tot);
   if (t < 12) To[t] = row;
   __syncthreads();
   if (t < 64) {

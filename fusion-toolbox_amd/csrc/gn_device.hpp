@@ -490,24 +490,43 @@ __device__ inline void block_partial(float (&acc)[kSlots], double* out,
 // BLOCK / 64 * kSlots doubles.
 constexpr int kSumBatch = 8;
 
+// One batch of sum_partials' loads: partials b0, b0 + BLOCK / 32, ... of
+// this thread's slot. Loads past the end re-read the last partial and are
+// dropped, so the batch issues without branches.
 template <int BLOCK>
-__device__ inline void sum_partials(const double* partials, int count,
-                                    double* sh, double* tot) {
+__device__ inline void load_partial_batch(const double* partials, int count,
+                                          int b0, double (&v)[kSumBatch]) {
+  constexpr int kG = BLOCK / kSlots;
+  const int slot = threadIdx.x & (kSlots - 1);
+#pragma unroll
+  for (int u = 0; u < kSumBatch; ++u) {
+    const int b = b0 + u * kG;
+    const double x = partials[int64_t(b < count ? b : count - 1) * kSlots + slot];
+    v[u] = b < count ? x : 0.0;
+  }
+}
+
+// sum_partials with the thread's first batch already loaded (by
+// load_partial_batch at b0 = threadIdx.x / 32), so a caller can issue other
+// loads behind it: loads return in issue order.
+template <int BLOCK>
+__device__ inline void sum_partials_from(double (&v)[kSumBatch],
+                                         const double* partials, int count,
+                                         double* sh, double* tot) {
   constexpr int kG = BLOCK / kSlots;
   constexpr int kW = BLOCK / 64;
   const int slot = threadIdx.x & (kSlots - 1);
   const int grp = threadIdx.x / kSlots;
+  // The first batch outside the loop (all of it dropped when grp >= count),
+  // so that its loads are unconditional and stay where the caller put them.
   double s = 0.0;
-  for (int b0 = grp; b0 < count; b0 += kSumBatch * kG) {
-    double v[kSumBatch];
 #pragma unroll
-    for (int u = 0; u < kSumBatch; ++u) {
-      // Loads past the end re-read the last partial and are dropped, so
-      // the batch issues without branches.
-      const int b = b0 + u * kG;
-      const double x = partials[int64_t(b < count ? b : count - 1) * kSlots + slot];
-      v[u] = b < count ? x : 0.0;
-    }
+  for (int w = kSumBatch / 2; w > 0; w /= 2)
+#pragma unroll
+    for (int u = 0; u < w; ++u) v[u] = v[u] + v[u + w];
+  s += v[0];
+  for (int b0 = grp + kSumBatch * kG; b0 < count; b0 += kSumBatch * kG) {
+    load_partial_batch<BLOCK>(partials, count, b0, v);
 #pragma unroll
     for (int w = kSumBatch / 2; w > 0; w /= 2)
 #pragma unroll
@@ -529,6 +548,14 @@ __device__ inline void sum_partials(const double* partials, int count,
     tot[threadIdx.x] = g[0];
   }
   __syncthreads();
+}
+
+template <int BLOCK>
+__device__ inline void sum_partials(const double* partials, int count,
+                                    double* sh, double* tot) {
+  double v[kSumBatch];
+  load_partial_batch<BLOCK>(partials, count, int(threadIdx.x) / kSlots, v);
+  sum_partials_from<BLOCK>(v, partials, count, sh, tot);
 }
 
 // ---------------------------------------------------------------------------
