@@ -390,7 +390,8 @@ typedef struct stb_icp_state {
 size_t stb_icp_state_bytes(void);
 
 /* Bytes of workspace needed for a level of n_src source and height*width
- * target pixels (state, two correspondence maps, block partials). */
+ * target pixels (state, two correspondence maps, block partials, the loop's
+ * solve record). No counterpart in the reference. */
 size_t stb_icp_workspace_bytes(int64_t n_src, int32_t height, int32_t width);
 
 /* Resets the state to the 4x4 row-major FP64 pose `transform0` (device
