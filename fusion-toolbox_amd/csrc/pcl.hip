@@ -170,10 +170,29 @@ __global__ void __launch_bounds__(kBlock)
   int cx, cy, cz;
   grid_coord(p, lox, loy, loz, inv, cx, cy, cz);
   if (cx != INT_MIN) {
+    // A neighbour cell whose nearest face lies at least the bound away holds
+    // no candidate (the test d2 < bound2 is strict): in units of cells, the
+    // query sits at r in [0, 1) of its cell, a face of the cell at offset -1
+    // is r away, one at +1 is 1 - r. Each face moves towards the query by
+    // `slack` cells first, more than the rounding of any point's cell
+    // coordinate, so a skipped cell never holds a point the scan would take.
+    const T rx = (p.x - lox) * inv - T(cx), ry = (p.y - loy) * inv - T(cy),
+            rz = (p.z - loz) * inv - T(cz);
+    const T eps = sizeof(T) == 4 ? T(1.1920929e-7) : T(2.220446049250313e-16);
+    const T slack = T(1e-3) + T(8) * eps * T(max(nx, max(ny, nz)));
+    const T b2 = bound2 * inv * inv;
+    auto gap2 = [slack](T r, int off) {
+      const T g = (off < 0 ? r : (off > 0 ? T(1) - r : T(0))) - slack;
+      return g > T(0) ? g * g : T(0);
+    };
+    const T gl = gap2(rx, -1), gr = gap2(rx, 1);
     for (int z = max(cz - 1, 0); z <= min(cz + 1, nz - 1); ++z)
       for (int y = max(cy - 1, 0); y <= min(cy + 1, ny - 1); ++y) {
+        const T gyz = gap2(rz, z - cz) + gap2(ry, y - cy);
+        if (!(gyz < b2)) continue;
         // The x-neighbours of a row are adjacent cells: one run of entries.
-        const int x0 = max(cx - 1, 0), x1 = min(cx + 1, nx - 1);
+        const int x0 = max(cx - (gyz + gl < b2 ? 1 : 0), 0);
+        const int x1 = min(cx + (gyz + gr < b2 ? 1 : 0), nx - 1);
         if (x0 > x1) continue;
         const int64_t row = (int64_t(z) * ny + y) * nx;
         const int64_t lo = start[row + x0], hi = start[row + x1 + 1];
