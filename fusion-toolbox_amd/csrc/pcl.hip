@@ -121,21 +121,21 @@ struct GridEntry;
 template <>
 struct GridEntry<float> {
   __device__ static inline void load(const float* e, int64_t a, V3<float>& p,
-                                     int64_t& j) {
+                                     int32_t& j) {
     const float4 v = reinterpret_cast<const float4*>(e)[a];
     p = V3<float>{v.x, v.y, v.z};
-    j = int64_t(__float_as_int(v.w));
+    j = __float_as_int(v.w);
   }
 };
 
 template <>
 struct GridEntry<double> {
   __device__ static inline void load(const double* e, int64_t a, V3<double>& p,
-                                     int64_t& j) {
+                                     int32_t& j) {
     const double2 u = reinterpret_cast<const double2*>(e)[2 * a];
     const double2 v = reinterpret_cast<const double2*>(e)[2 * a + 1];
     p = V3<double>{u.x, u.y, v.x};
-    j = int64_t(v.y);
+    j = int32_t(v.y);
   }
 };
 
@@ -160,12 +160,14 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t q = order ? order[t] : t;
   const V3<T> p = ld3(qry, q);
   const T inf = T(INFINITY);
+  // Indices in 32 bits (m < 2^31): half the registers and compares of the
+  // insertion network.
   T d[K];
-  int64_t id[K];
+  int32_t id[K];
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     d[s] = inf;
-    id[s] = m;
+    id[s] = int32_t(m);
   }
   int cx, cy, cz;
   grid_coord(p, lox, loy, loz, inv, cx, cy, cz);
@@ -201,7 +203,7 @@ __global__ void __launch_bounds__(kBlock)
         // distance, which never inserts.
         for (int64_t a = lo; a < hi; a += 4) {
           T d2s[4];
-          int64_t js[4];
+          int32_t js[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const bool in = a + u < hi;
@@ -213,7 +215,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const T d2 = d2s[u];
-            const int64_t j = js[u];
+            const int32_t j = js[u];
             if (!(d2 < bound2)) continue;
             if (!(d2 < d[K - 1] || (d2 == d[K - 1] && j < id[K - 1]))) continue;
 #pragma unroll
@@ -235,7 +237,7 @@ __global__ void __launch_bounds__(kBlock)
   for (int s = 0; s < K; ++s) {
     if (s < k) {
       out_d[q * k + s] = d[s];
-      out_i[q * k + s] = id[s];
+      out_i[q * k + s] = int64_t(id[s]);
     }
   }
 }
@@ -536,8 +538,9 @@ extern "C" int stb_knn_grid(int32_t dtype, const void* entries,
   STB_REQUIRE(query && out_sq_dist && out_index && cell_start &&
                   (m == 0 || entries),
               "knn_grid: NULL argument");
-  STB_REQUIRE(dtype == STB_F64 || m < (int64_t(1) << 31),
-              "knn_grid: float32 entries hold 31-bit indices");
+  STB_REQUIRE(m < (int64_t(1) << 31),
+              "knn_grid: at most 2^31 - 1 targets (31-bit indices), got %lld",
+              (long long)m);
   GridArgs g{entries, cell_start, nx, ny, nz, {lo_x, lo_y, lo_z}, cell};
   STB_FLOAT_DISPATCH(dtype, knn_grid_dispatch, g, m, query, n, query_order, k,
                      distance_upper_bound, out_sq_dist, out_index,

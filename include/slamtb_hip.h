@@ -256,7 +256,8 @@ int stb_grid_cells(int32_t dtype, const void* points, int64_t m, double lo_x,
  * in which threads take the queries: ordered by grid cell (stb_grid_cells
  * of the queries, stably sorted), a wavefront's queries scan the same cells
  * and its candidate loads coalesce. Each query's row of the outputs is its
- * own either way. Same result as stb_knn (ties broken by original index). */
+ * own either way. Same result as stb_knn (ties broken by original index).
+ * m must be below 2^31 (the search keeps 31-bit indices). */
 int stb_knn_grid(int32_t dtype, const void* entries, const int64_t* cell_start,
                  int32_t nx, int32_t ny, int32_t nz,
                  double lo_x, double lo_y, double lo_z, double cell, int64_t m,
