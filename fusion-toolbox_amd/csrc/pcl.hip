@@ -139,15 +139,35 @@ struct GridEntry<double> {
   }
 };
 
+// Inserts (d2, j) into the (distance, index)-ordered top-K list (d, id) if
+// it is within the bound and precedes the list's last entry.
+template <typename T, int K>
+__device__ inline void topk_insert(T (&d)[K], int32_t (&id)[K], T d2, int32_t j,
+                                   T bound2) {
+  if (!(d2 < bound2)) return;
+  if (!(d2 < d[K - 1] || (d2 == d[K - 1] && j < id[K - 1]))) return;
+#pragma unroll
+  for (int s = K - 1; s > 0; --s) {
+    const bool shift = d[s - 1] > d2 || (d[s - 1] == d2 && id[s - 1] > j);
+    const bool here = !shift && (d[s] > d2 || (d[s] == d2 && id[s] > j));
+    d[s] = shift ? d[s - 1] : (here ? d2 : d[s]);
+    id[s] = shift ? id[s - 1] : (here ? j : id[s]);
+  }
+  if (d[0] > d2 || (d[0] == d2 && id[0] > j)) {
+    d[0] = d2;
+    id[0] = j;
+  }
+}
+
 // The same top-K as knn_kernel, over the 27 grid cells around the query
 // (cells are at least the distance bound wide, so they hold every target
 // within it). Candidates arrive out of index order, so ties compare
-// (distance, index) and the result equals the brute-force search's. Thread
-// t takes query order[t] (order NULL: query t): with the queries ordered by
-// grid cell, the lanes of a wavefront scan the same cells, so their
-// candidate loads coalesce into the same lines and their loops run the same
-// trip counts.
-template <typename T, int K>
+// (distance, index) and the result equals the brute-force search's. Lane
+// group t takes query order[t] (order NULL: query t): with the queries
+// ordered by grid cell, the lanes of a wavefront scan the same cells, so
+// their candidate loads coalesce into the same lines and their loops run the
+// same trip counts.
+template <typename T, int K, int L>
 __global__ void __launch_bounds__(kBlock)
     knn_grid_kernel(const T* __restrict__ entries, const int64_t* __restrict__ start,
                     int nx, int ny, int nz, T lox, T loy, T loz, T inv, int64_t m,
@@ -155,8 +175,14 @@ __global__ void __launch_bounds__(kBlock)
                     const int64_t* __restrict__ order, int k, T bound2,
                     T* __restrict__ out_d, int64_t* __restrict__ out_i) {
 #pragma clang fp contract(off)
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+  // L consecutive lanes share a query: lane `sub` scans candidates sub,
+  // sub + L, ... of each run and keeps its own top K; the lanes then merge
+  // their lists. The top K of the union of disjoint candidate sets under the
+  // total order (distance, index) is the same whichever lane saw what.
+  const int64_t gt = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t t = gt / L;
+  const int sub = int(gt % L);
+  if (t >= n) return;  // whole groups of L lanes: n * L threads are needed
   const int64_t q = order ? order[t] : t;
   const V3<T> p = ld3(qry, q);
   const T inf = T(INFINITY);
@@ -201,38 +227,37 @@ __global__ void __launch_bounds__(kBlock)
         // Candidates in batches of four whose loads issue together; a slot
         // past the run re-reads the run's last entry with an infinite
         // distance, which never inserts.
-        for (int64_t a = lo; a < hi; a += 4) {
+        for (int64_t a = lo + sub; a < hi; a += 4 * L) {
           T d2s[4];
           int32_t js[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            const bool in = a + u < hi;
+            const bool in = a + u * L < hi;
             V3<T> t;
-            GridEntry<T>::load(entries, in ? a + u : hi - 1, t, js[u]);
+            GridEntry<T>::load(entries, in ? a + u * L : hi - 1, t, js[u]);
             const T d2 = sq_dist(p, t.x, t.y, t.z);
             d2s[u] = in ? d2 : inf;
           }
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const T d2 = d2s[u];
-            const int32_t j = js[u];
-            if (!(d2 < bound2)) continue;
-            if (!(d2 < d[K - 1] || (d2 == d[K - 1] && j < id[K - 1]))) continue;
-#pragma unroll
-            for (int s = K - 1; s > 0; --s) {
-              const bool shift = d[s - 1] > d2 || (d[s - 1] == d2 && id[s - 1] > j);
-              const bool here = !shift && (d[s] > d2 || (d[s] == d2 && id[s] > j));
-              d[s] = shift ? d[s - 1] : (here ? d2 : d[s]);
-              id[s] = shift ? id[s - 1] : (here ? j : id[s]);
-            }
-            if (d[0] > d2 || (d[0] == d2 && id[0] > j)) {
-              d[0] = d2;
-              id[0] = j;
-            }
-          }
+          for (int u = 0; u < 4; ++u) topk_insert<T, K>(d, id, d2s[u], js[u], bound2);
         }
       }
   }
+  // Butterfly merge: each step snapshots the partner's list, then inserts
+  // it (the lists cover disjoint candidates).
+#pragma unroll
+  for (int off = 1; off < L; off *= 2) {
+    T pd[K];
+    int32_t pj[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      pd[s] = __shfl_xor(d[s], off, 64);
+      pj[s] = __shfl_xor(id[s], off, 64);
+    }
+#pragma unroll
+    for (int s = 0; s < K; ++s) topk_insert<T, K>(d, id, pd[s], pj[s], bound2);
+  }
+  if (sub != 0) return;
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     if (s < k) {
@@ -431,15 +456,31 @@ struct GridArgs {
   double lo[3], cell;
 };
 
+// Lanes per query of the grid search: with fewer queries than it takes to
+// keep the CUs' memory pipelines busy one lane per query, each query's
+// candidates are split over kKnnLanes lanes.
+constexpr int kKnnLanes = 4;
+constexpr int64_t kKnnOneLaneQueries = int64_t(1) << 20;
+
+template <typename T, int K, int L>
+void launch_knn_grid_lanes(const GridArgs& g, int64_t m, const void* qry,
+                           int64_t n, const int64_t* order, int k, double bound,
+                           void* d, int64_t* i, hipStream_t s) {
+  hipLaunchKernelGGL((knn_grid_kernel<T, K, L>), dim3(grid_for(n * L)),
+                     dim3(kBlock), 0, s, static_cast<const T*>(g.entries),
+                     g.start, g.nx, g.ny, g.nz, T(g.lo[0]), T(g.lo[1]),
+                     T(g.lo[2]), T(1) / T(g.cell), m, static_cast<const T*>(qry),
+                     n, order, k, T(bound) * T(bound), static_cast<T*>(d), i);
+}
+
 template <typename T, int K>
 void launch_knn_grid(const GridArgs& g, int64_t m, const void* qry, int64_t n,
                      const int64_t* order, int k, double bound, void* d,
                      int64_t* i, hipStream_t s) {
-  hipLaunchKernelGGL((knn_grid_kernel<T, K>), dim3(grid_for(n)), dim3(kBlock), 0,
-                     s, static_cast<const T*>(g.entries), g.start, g.nx,
-                     g.ny, g.nz, T(g.lo[0]), T(g.lo[1]), T(g.lo[2]),
-                     T(1) / T(g.cell), m, static_cast<const T*>(qry), n, order,
-                     k, T(bound) * T(bound), static_cast<T*>(d), i);
+  if (n <= kKnnOneLaneQueries)
+    launch_knn_grid_lanes<T, K, kKnnLanes>(g, m, qry, n, order, k, bound, d, i, s);
+  else
+    launch_knn_grid_lanes<T, K, 1>(g, m, qry, n, order, k, bound, d, i, s);
 }
 
 template <typename T>
