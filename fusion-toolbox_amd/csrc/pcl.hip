@@ -460,6 +460,9 @@ struct GridArgs {
 // keep the CUs' memory pipelines busy one lane per query, each query's
 // candidates are split over kKnnLanes lanes.
 constexpr int kKnnLanes = 4;
+// The kNN launches give each query its own thread (no grid-stride loop):
+// grid_for's 2^20 blocks of kBlock threads bound the queries per call.
+constexpr int64_t kMaxKnnQueries = int64_t(1) << 28;
 constexpr int64_t kKnnOneLaneQueries = int64_t(1) << 20;
 
 template <typename T, int K, int L>
@@ -531,6 +534,9 @@ extern "C" int stb_knn(int32_t dtype, const void* target, int64_t m,
               "knn: dtype must be float32 or float64");
   STB_REQUIRE(m >= 0 && n >= 0, "knn: bad sizes");
   STB_REQUIRE(k >= 1 && k <= 16, "knn: k must be in [1, 16], got %d", k);
+  STB_REQUIRE(n <= kMaxKnnQueries,
+              "knn: at most 2^28 queries per call (one thread each), got %lld",
+              (long long)n);
   STB_REQUIRE(distance_upper_bound > 0.0, "knn: distance_upper_bound must be > 0");
   if (n == 0) return STB_OK;
   STB_REQUIRE(query && out_sq_dist && out_index && (m == 0 || target),
@@ -569,6 +575,9 @@ extern "C" int stb_knn_grid(int32_t dtype, const void* entries,
   STB_REQUIRE(dtype == STB_F32 || dtype == STB_F64,
               "knn_grid: dtype must be float32 or float64");
   STB_REQUIRE(k >= 1 && k <= 16, "knn_grid: k must be in [1, 16], got %d", k);
+  STB_REQUIRE(n <= kMaxKnnQueries,
+              "knn_grid: at most 2^28 queries per call (one thread each), got %lld",
+              (long long)n);
   STB_REQUIRE(m >= 0 && n >= 0 && nx > 0 && ny > 0 && nz > 0,
               "knn_grid: bad sizes");
   STB_REQUIRE(std::isfinite(distance_upper_bound) && distance_upper_bound > 0.0 &&

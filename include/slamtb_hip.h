@@ -223,11 +223,6 @@ int stb_rigid_transform_normals(int32_t dtype, const void* matrix,
                                 const void* normals, int64_t n, void* out,
                                 void* stream);
 
-/* Backward of out = R p + t (no native counterpart: the reference
- * differentiates its torch matmul, slamtb/camera.py:333-337):
- * dx_points[n,3] = R^T dy (skipped when NULL) and d_matrix[3,4] =
- * sum_i dy_i [p_i; 1]^T, summed in FP64 in a fixed order. workspace is device
- * memory of at least stb_rigid_backward_workspace_bytes() bytes. */
 /* Exact k-nearest-neighbour search, the device replacement of the host
  * k-d tree query of KDTreeLayer.query (slamtb/spatial/kdtree_layer.py:40-60,
  * pykdtree KDTree.query with sqr_dists=True): for each of n query points the
@@ -236,7 +231,7 @@ int stb_rigid_transform_normals(int32_t dtype, const void* matrix,
  * (pass +inf for no bound), sorted by squared distance and then by target
  * index; out_sq_dist [n,k] and out_index [n,k] hold (inf, m) in unused
  * slots. Exact brute force (the reference's k-d tree searched approximately,
- * eps = 0.5). */
+ * eps = 0.5). At most 2^28 queries per call. */
 int stb_knn(int32_t dtype, const void* target, int64_t m, const void* query,
             int64_t n, int32_t k, double distance_upper_bound,
             void* out_sq_dist, int64_t* out_index, void* stream);
@@ -257,7 +252,8 @@ int stb_grid_cells(int32_t dtype, const void* points, int64_t m, double lo_x,
  * of the queries, stably sorted), a wavefront's queries scan the same cells
  * and its candidate loads coalesce. Each query's row of the outputs is its
  * own either way. Same result as stb_knn (ties broken by original index).
- * m must be below 2^31 (the search keeps 31-bit indices). */
+ * m must be below 2^31 (the search keeps 31-bit indices); at most 2^28
+ * queries per call. */
 int stb_knn_grid(int32_t dtype, const void* entries, const int64_t* cell_start,
                  int32_t nx, int32_t ny, int32_t nz,
                  double lo_x, double lo_y, double lo_z, double cell, int64_t m,
@@ -316,6 +312,11 @@ int stb_voxel_merge(const float* points, const float* normals,
                     uint8_t* out_colors, float* out_confidences, float* out_radii,
                     int32_t* out_times, float* out_features, void* stream);
 
+/* Backward of out = R p + t (no native counterpart: the reference
+ * differentiates its torch matmul, slamtb/camera.py:333-337):
+ * dx_points[n,3] = R^T dy (skipped when NULL) and d_matrix[3,4] =
+ * sum_i dy_i [p_i; 1]^T, summed in FP64 in a fixed order. workspace is device
+ * memory of at least stb_rigid_backward_workspace_bytes() bytes. */
 size_t stb_rigid_backward_workspace_bytes(void);
 int stb_rigid_transform_backward(int32_t dtype, const void* matrix,
                                  const void* points, int64_t n, const void* dy,

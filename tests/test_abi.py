@@ -109,3 +109,21 @@ def test_batch_plan_layout_on_the_host():
     assert np.array_equal(pairs, np.repeat(np.arange(3), tiles))
     # tile_px must be a multiple of the 256-lane workgroup
     assert lib.stb_icp_batch_plan(3, levels, 100, None, 0, ctypes.byref(sizes)) != 0
+
+
+def test_knn_size_limits_are_errors():
+    """The neighbour searches give each query its own thread and the grid
+    search keeps 31-bit indices: sizes past either limit are refused on the
+    host with STB_ERR_INVALID before anything reaches the device."""
+    lib = _lib.lib()
+    fake = ctypes.c_void_p(256)  # never dereferenced: the checks fail first
+    i64 = ctypes.cast(fake, ctypes.POINTER(ctypes.c_int64))
+    too_many = (1 << 28) + 1
+    rc = lib.stb_knn(0, fake, 10, fake, too_many, 8, math.inf, fake, i64, None)
+    assert rc == 1 and b"2^28" in lib.stb_last_error()
+    rc = lib.stb_knn_grid(0, fake, i64, 2, 2, 2, 0.0, 0.0, 0.0, 0.1, 10, fake,
+                          too_many, None, 8, 0.1, fake, i64, None)
+    assert rc == 1 and b"2^28" in lib.stb_last_error()
+    rc = lib.stb_knn_grid(0, fake, i64, 2, 2, 2, 0.0, 0.0, 0.0, 0.1, 1 << 31,
+                          fake, 10, None, 8, 0.1, fake, i64, None)
+    assert rc == 1 and b"31-bit" in lib.stb_last_error()
