@@ -794,8 +794,9 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not "
-              "supported (independent frame pairs shard with no exchange; "
-              "see the multi-GPU note in DESIGN.md)", file=sys.stderr)
+              "supported (independent frame pairs shard with no exchange "
+              "between GPUs: one process per GPU, each with its own pairs)",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     from slamtb.registration._fused import runner_for

@@ -15,7 +15,7 @@ output convention as the pybind11 bindings:
 * ``NearestNeighborsOp`` -- src/nearest_neighbors.cpp:6-12
 
 Differences, all stricter or documented: tensors must live on a HIP device
-(the reference also had a CPU instantiation; this build has no CPU path),
+(the reference also had a CPU instantiation; this library has no CPU path),
 output tensors must be contiguous, and ``kcam``/``rt_cam`` are cast to the
 source dtype instead of failing the accessor type check.
 """

@@ -237,7 +237,7 @@ def require_gpu(*tensors, what="slamtb"):
             continue
         if not t.is_cuda:
             raise RuntimeError(
-                f"Check failed: {what}: Expected a gpu tensor (this build "
+                f"Check failed: {what}: Expected a gpu tensor (this library "
                 "runs on HIP devices only)")
         if dev is None:
             dev = t.device
