@@ -94,17 +94,18 @@ __device__ inline LevelView<S> view_of(const stb_icp_level& L) {
 // ---------------------------------------------------------------------------
 //
 // The fused correspondence and Jacobian passes read their inputs through
-// raw buffer resources over whole arrays (points, normals and masks of both
-// frames, the [C, H, W] target and [C, n] source features, the map): a load
-// is a 32-bit byte offset (an array's channel in a scalar register, the
-// element in a vector register, small constant steps as immediates) instead
-// of a 64-bit address of its own, and a load past the end of the array
-// returns 0 instead of faulting. The bilinear taps therefore need no
-// clamping: a tap past the right edge reads the next row's first texel, one
-// past the bottom the next channel's (or 0 past the last), and bil_mix
-// replaces both by zero exactly as it replaces the clamped re-reads of
-// bil_taps. Every array must hold fewer than 2^31 bytes (checked on the
-// host).
+// raw buffer resources (points, normals and masks of both frames, each
+// channel of the [C, H, W] target features, the [C, n] source features, the
+// map): a load is a 32-bit byte offset in a vector register (small constant
+// steps as immediates) instead of a 64-bit address of its own, and a load
+// whose vector offset lies past the end of the resource returns 0 instead of
+// faulting. The scalar offset operand is not range-checked by the hardware,
+// so every load here passes 0 there and puts the whole offset in the vector
+// operand. The bilinear taps therefore need no clamping: each target channel
+// has a resource of its own, a tap past the right edge reads the next row's
+// first texel and one past the bottom row returns 0, and bil_mix replaces
+// both by zero exactly as it replaces the clamped re-reads of bil_taps.
+// Every array must hold fewer than 2^31 bytes (checked on the host).
 constexpr int kRawBufferWord3 = 0x00020000;  // raw 32-bit data format, CDNA
 
 // Whether a level's arrays fit those 32-bit byte offsets.
@@ -250,16 +251,18 @@ __device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
 // loads issue in one batch: a pixel costs two memory round trips. A pixel
 // that does not own its target adds exact zeros (selected, never multiplied
 // in, so NaN or inf in its unused data cannot leak into the sums).
+// The four bilinear taps of one sample from the resource of one [H, W]
+// channel.
 template <typename S>
-__device__ inline void res_taps(__amdgpu_buffer_rsrc_t r, int soff, int W,
+__device__ inline void res_taps(__amdgpu_buffer_rsrc_t r, int W,
                                 const Bil<float>& b, float t[4]) {
   constexpr int es = int(sizeof(S));
   const int o = (b.vi * W + b.ui) * es;
   const int d = o + W * es;
-  t[0] = res_ld<S>(r, o, soff);
-  t[1] = res_ld<S>(r, o + es, soff);
-  t[2] = res_ld<S>(r, d, soff);
-  t[3] = res_ld<S>(r, d + es, soff);
+  t[0] = res_ld<S>(r, o, 0);
+  t[1] = res_ld<S>(r, o + es, 0);
+  t[2] = res_ld<S>(r, d, 0);
+  t[3] = res_ld<S>(r, d + es, 0);
 }
 
 // feature_terms<float, S, S, true, NB> (icp_math.hpp) with its taps and
@@ -271,10 +274,10 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
                                           float j[6], float& rlast) {
 #pragma clang fp contract(off)
   constexpr int es = int(sizeof(S));
-  const __amdgpu_buffer_rsrc_t rt = make_res(L.tf, int64_t(L.C) * L.HW);
+  // The source value of channel c is element c * n + idx of one resource
+  // over all channels (idx < n, so the offset stays inside the array).
   const __amdgpu_buffer_rsrc_t rs = make_res(L.sf, int64_t(L.C) * L.n);
-  const int tch = int(L.HW) * es;  // bytes per target channel
-  const int sch = int(L.n) * es;   // bytes per source channel
+  const int sch = int(L.n) * es;  // bytes per source channel
   const int so = int(idx) * es;
   const int H = L.H, W = L.W, C = L.C;
   const float h = 0.05f;
@@ -285,14 +288,14 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
   const Bil<float> bv0 = make_bil(u, v + h);
   const Bil<float> bv1 = make_bil(u, fmax(v - h, 0.0f));
   // The last channel's five samples first (see feature_terms).
-  const int lt = (C - 1) * tch;
+  const __amdgpu_buffer_rsrc_t rl = make_res(L.tf + int64_t(C - 1) * L.HW, L.HW);
   float t0[4], t1[4], t2[4], t3[4], t4[4];
-  res_taps<S>(rt, lt, W, c0, t0);
-  res_taps<S>(rt, lt, W, bu0, t1);
-  res_taps<S>(rt, lt, W, bu1, t2);
-  res_taps<S>(rt, lt, W, bv0, t3);
-  res_taps<S>(rt, lt, W, bv1, t4);
-  const float flast = res_ld<S>(rs, so, (C - 1) * sch);
+  res_taps<S>(rl, W, c0, t0);
+  res_taps<S>(rl, W, bu0, t1);
+  res_taps<S>(rl, W, bu1, t2);
+  res_taps<S>(rl, W, bv0, t3);
+  res_taps<S>(rl, W, bv1, t4);
+  const float flast = res_ld<S>(rs, so + (C - 1) * sch, 0);
   const float last = bil_mix(H, W, c0, t0);
   const float su0 = bil_mix(H, W, bu0, t1), su1 = bil_mix(H, W, bu1, t2);
   const float sv0 = bil_mix(H, W, bv0, t3), sv1 = bil_mix(H, W, bv1, t4);
@@ -303,8 +306,8 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
       const int c = c0i + k < last_c ? c0i + k : last_c;
-      res_taps<S>(rt, c * tch, W, c0, taps[k]);
-      fs[k] = res_ld<S>(rs, so, c * sch);
+      res_taps<S>(make_res(L.tf + int64_t(c) * L.HW, L.HW), W, c0, taps[k]);
+      fs[k] = res_ld<S>(rs, so + c * sch, 0);
     }
 #pragma unroll
     for (int k = 0; k < NB; ++k) {

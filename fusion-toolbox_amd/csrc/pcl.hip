@@ -16,6 +16,8 @@
 // the same result.
 #include "stb_common.hpp"
 
+#include <atomic>
+
 namespace stb {
 namespace {
 
@@ -463,7 +465,11 @@ constexpr int kKnnLanes = 4;
 // The kNN launches give each query its own thread (no grid-stride loop):
 // grid_for's 2^20 blocks of kBlock threads bound the queries per call.
 constexpr int64_t kMaxKnnQueries = int64_t(1) << 28;
-constexpr int64_t kKnnOneLaneQueries = int64_t(1) << 20;
+// Above this many queries per call each query takes one lane (enough
+// threads to fill the CUs without the split). Settable for tests through
+// stb_knn_grid_set_lane_threshold, so that small calls can take the
+// one-lane instantiation too.
+std::atomic<int64_t> g_knn_one_lane_queries{int64_t(1) << 20};
 
 template <typename T, int K, int L>
 void launch_knn_grid_lanes(const GridArgs& g, int64_t m, const void* qry,
@@ -480,7 +486,7 @@ template <typename T, int K>
 void launch_knn_grid(const GridArgs& g, int64_t m, const void* qry, int64_t n,
                      const int64_t* order, int k, double bound, void* d,
                      int64_t* i, hipStream_t s) {
-  if (n <= kKnnOneLaneQueries)
+  if (n <= g_knn_one_lane_queries.load(std::memory_order_relaxed))
     launch_knn_grid_lanes<T, K, kKnnLanes>(g, m, qry, n, order, k, bound, d, i, s);
   else
     launch_knn_grid_lanes<T, K, 1>(g, m, qry, n, order, k, bound, d, i, s);
@@ -717,4 +723,8 @@ extern "C" int stb_voxel_merge(const float* points, const float* normals,
                      out_points, out_normals, out_colors, out_confidences,
                      out_radii, out_times, out_features);
   return check_launch("voxel_merge");
+}
+
+extern "C" int64_t stb_knn_grid_set_lane_threshold(int64_t max_split_queries) {
+  return g_knn_one_lane_queries.exchange(max_split_queries < 0 ? 0 : max_split_queries);
 }

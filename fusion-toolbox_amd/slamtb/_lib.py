@@ -135,6 +135,7 @@ _SIGNATURES = {
     "stb_knn_grid": (ctypes.c_int, [
         _i32, _vp, _vp, _i32, _i32, _i32, _f64, _f64, _f64, _f64, _i64, _vp,
         _i64, _vp, _i32, _f64, _vp, _vp, _vp]),
+    "stb_knn_grid_set_lane_threshold": (_i64, [_i64]),
     "stb_nn_features_forward": (ctypes.c_int, [
         _i32, _vp, _vp, _i64, _i32, _vp, _i32, _i64, _vp, _vp]),
     "stb_nn_epsilon_distances": (ctypes.c_int, [

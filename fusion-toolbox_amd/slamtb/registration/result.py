@@ -31,7 +31,8 @@ class RegistrationVerifier:
 
     The reference calls ``torch.eig``, which current torch no longer has;
     this uses ``torch.linalg.eigvals`` and keeps the real parts, which is the
-    same first column ``torch.eig`` returned.
+    same first column ``torch.eig`` returned. As in the reference the
+    information matrix is inverted in its own dtype.
     """
 
     def __init__(self, match_ratio_threshold=6e-1,
@@ -44,8 +45,7 @@ class RegistrationVerifier:
         if result.residual > self.residual_threshhold:
             return False
         if result.information is not None:
-            info = result.information.double()
-            eigvals = torch.linalg.eigvals(info.inverse()).real
+            eigvals = torch.linalg.eigvals(result.information.inverse()).real
             if torch.any(eigvals > self.covariance_max_threshold):
                 return False
         if result.match_ratio < self.match_ratio_threshold:

@@ -27,20 +27,41 @@ class KDTreeLayer(torch.autograd.Function):
 
     _gradcheck = False
     _grid = None
+    _key = None
+
+    @staticmethod
+    def _target_key(target_xyz):
+        """Identity of the target's current contents: its storage, shape,
+        dtype and version counter (bumped by every in-place write to the
+        tensor or a view of it)."""
+        return (target_xyz.data_ptr(), tuple(target_xyz.shape), target_xyz.dtype,
+                target_xyz._version)
+
+    @staticmethod
+    def _refresh():
+        """Re-read the target (and drop the search grid) when it is another
+        tensor or has been written in place since the last read."""
+        key = KDTreeLayer._target_key(KDTreeLayer.target_xyz)
+        if key != KDTreeLayer._key:
+            KDTreeLayer._key = key
+            KDTreeLayer.tree = KDTreeLayer.target_xyz.detach().reshape(-1, 3).contiguous()
+            KDTreeLayer._grid = None
 
     @staticmethod
     def setup(target_xyz):
-        if KDTreeLayer.target_xyz is not target_xyz:
-            KDTreeLayer.target_xyz = target_xyz
-            KDTreeLayer.tree = target_xyz.detach().reshape(-1, 3).contiguous()
-            KDTreeLayer._grid = None
+        KDTreeLayer.target_xyz = target_xyz
+        KDTreeLayer._refresh()
         return KDTreeLayer.apply
 
     @staticmethod
     def query(xyz, k=1, distance_upper_bound=math.inf):
         """Returns the [N] mask of queries with at least one neighbour;
         ``last_query`` = (squared distances [N, k], index [N, k]). A finite
-        bound reuses the target grid built for the first such query."""
+        bound reuses the target grid built for the first such query while
+        the target is unchanged (the reference rebuilt its k-d tree in every
+        setup; here the grid is rebuilt when the target is another tensor or
+        was written in place)."""
+        KDTreeLayer._refresh()
         tree = KDTreeLayer.tree
         q = xyz.detach().reshape(-1, 3)
         grid = KDTreeLayer._grid

@@ -261,6 +261,13 @@ int stb_knn_grid(int32_t dtype, const void* entries, const int64_t* cell_start,
                  int32_t k, double distance_upper_bound, void* out_sq_dist,
                  int64_t* out_index, void* stream);
 
+/* Test hook of stb_knn_grid, no counterpart in the reference: calls with at
+ * most max_split_queries queries split each query's candidates over four
+ * lanes, larger calls give each query one lane (default 2^20). Returns the
+ * previous value. Both give the same results; tests lower it to run the
+ * one-lane form on small inputs. */
+int64_t stb_knn_grid_set_lane_threshold(int64_t max_split_queries);
+
 /* NearestNeighborsOp.forward (src/nearest_neighbors.cpp:7,
  * src/nearest_neighbors.cu:13-79): out_features [channels, n] = sum over the
  * neighbours of features[:, index] * w, w = (1 / (1 + d)) / sum of 1 / (1 + d)

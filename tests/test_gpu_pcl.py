@@ -54,6 +54,57 @@ def test_knn_bit_exact(dtype, k, bound):
     assert ri[5, 0] == 100
 
 
+@pytest.mark.parametrize("k,bound", [(1, 0.08), (8, 0.12), (16, 0.3)])
+def test_knn_grid_one_lane_form_bit_exact(k, bound):
+    """The one-lane grid search (taken above 2^20 queries per call) forced on
+    a small call: the same indices and distances as the oracle's brute
+    force, as the four-lane form gives."""
+    from slamtb import _lib
+    rng = np.random.default_rng(100 + k)
+    tg = rng.uniform(0, 1, size=(3000, 3)).astype(np.float32)
+    tg[7] = tg[8] = tg[9]
+    q = rng.uniform(0, 1, size=(1500, 3)).astype(np.float32)
+    q[3] = tg[8]
+    rd, ri = orc.knn(tg, q, k, bound)
+    lib = _lib.lib()
+    prev = lib.stb_knn_grid_set_lane_threshold(0)
+    try:
+        d, i = knn(t(tg), t(q), k, bound)
+    finally:
+        lib.stb_knn_grid_set_lane_threshold(prev)
+    assert torch.equal(i.cpu(), torch.from_numpy(ri))
+    assert np.array_equal(d.cpu().numpy(), rd)
+    d4, i4 = knn(t(tg), t(q), k, bound)
+    assert torch.equal(i4.cpu(), i.cpu()) and torch.equal(d4.cpu(), d.cpu())
+
+
+def test_kdtree_layer_sees_in_place_target_updates():
+    """A target written in place between two queries is searched as it is
+    now (the cached search grid is rebuilt): the neighbours equal the
+    brute-force search over the new points."""
+    rng = np.random.default_rng(5)
+    target = t(rng.uniform(0, 1, size=(2000, 3)).astype(np.float32))
+    src = t(rng.uniform(0, 1, size=(500, 3)).astype(np.float32))
+    KDTreeLayer.setup(target)
+    KDTreeLayer.query(src, 8, 0.1)
+    first = KDTreeLayer.last_query[1].clone()
+    with torch.no_grad():
+        target.mul_(0.5).add_(0.25)  # same tensor, new contents
+    KDTreeLayer.setup(target)
+    KDTreeLayer.query(src, 8, 0.1)
+    dist, index = KDTreeLayer.last_query
+    rd, ri = orc.knn(target.cpu().numpy(), src.cpu().numpy(), 8, 0.1)
+    assert torch.equal(index.cpu(), torch.from_numpy(ri))
+    assert np.array_equal(dist.cpu().numpy(), rd)
+    assert not torch.equal(first, index)
+    # Without a new setup the query still sees a later in-place write.
+    with torch.no_grad():
+        target.add_(0.01)
+    KDTreeLayer.query(src, 8, 0.1)
+    rd, ri = orc.knn(target.cpu().numpy(), src.cpu().numpy(), 8, 0.1)
+    assert torch.equal(KDTreeLayer.last_query[1].cpu(), torch.from_numpy(ri))
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_nn_feature_ops_bit_exact(dtype):
     rng = np.random.default_rng(7)

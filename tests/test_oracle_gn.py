@@ -72,3 +72,37 @@ def test_residual_and_weights_follow_the_reference_combination():
     count = max(gc, fc)
     expect = (fw * f2.sum(dtype=np.float64) + gw * g2.sum(dtype=np.float64)) / count
     assert abs(res["residual"] - expect) <= 1e-12 * expect
+
+
+def test_literal_one_to_point_one_weights_are_unstable_and_ten_to_one_are_not():
+    """Why the benchmark writes the 1 : 0.1 geometric : feature weighting as
+    10 : 1 (the form of slamtb/registration/preset.py:10). The reference
+    combines the terms with JtJ scaled by w^2 and Jr by w (icp.py:221-234),
+    so the two forms are not equivalent: with (1, 0.1) the feature term's
+    step is ten times too long for its curvature and the iteration is
+    chaotic -- the oracle's FP32 and FP64 per-pixel runs part by 1e-3 or
+    more after 10 iterations and end farther from the true motion -- while
+    with (10, 1) they stay within a few 1e-6 of each other."""
+    k = synthetic.intrinsics(160, 120).astype(np.float32)
+    for seed in (1, 2):
+        pr = synthetic.make_pair(seed, 160, 120, kcam=k, channels=3)
+        sm, tm = orc.erode(pr["src_depth"] > 0), orc.erode(pr["tgt_depth"] > 0)
+        sp, tp = orc.backproject(pr["src_depth"], k), orc.backproject(pr["tgt_depth"], k)
+        sn, tn = orc.normals(sp, sm), orc.normals(tp, tm)
+        out = {}
+        for gw, fw in ((1.0, 0.1), (10.0, 1.0)):
+            runs = [orc.icp_estimate(k, sp, sn, sm, tp, tn, tm, 10,
+                                     src_feats=pr["src_feats"],
+                                     tgt_feats=pr["tgt_feats"], geom_weight=gw,
+                                     feat_weight=fw, distance_threshold=0.1,
+                                     normals_angle_thresh=math.radians(20),
+                                     feat_residual_thresh=2.75, dtype=dt)
+                    for dt in (np.float32, np.float64)]
+            out[(gw, fw)] = (orc.pose_error(runs[0]["transform"], runs[1]["transform"]),
+                             orc.pose_error(runs[1]["transform"], pr["T_gt"]))
+        (lit, lit_gt), (pre, pre_gt) = out[(1.0, 0.1)], out[(10.0, 1.0)]
+        print(seed, "1:0.1 fp32-vs-fp64", lit, "vs truth", lit_gt,
+              "| 10:1", pre, "vs truth", pre_gt)
+        assert max(lit) > 1e-3
+        assert max(pre) < 1e-5
+        assert lit_gt[1] > pre_gt[1]

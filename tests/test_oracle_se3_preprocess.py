@@ -95,6 +95,114 @@ def test_bilateral_constant_ramp_and_isolated_pixel():
     assert set(np.unique(out)) <= {1499, 1500}
 
 
+def _box_mean(depth, mask, half):
+    """Mean of the valid pixels of the (2 half + 1)^2 window around each
+    valid pixel (0 where the centre is masked out), in FP64."""
+    H, W = depth.shape
+    out = np.zeros((H, W))
+    for r in range(H):
+        for c in range(W):
+            if not mask[r, c]:
+                continue
+            win = (slice(max(r - half, 0), r + half + 1),
+                   slice(max(c - half, 0), c + half + 1))
+            out[r, c] = depth[win][mask[win]].astype(np.float64).mean()
+    return out
+
+
+@pytest.mark.parametrize("filter_width,half", [(6, 3), (5, 2), (4, 2), (2, 1)])
+def test_bilateral_wide_sigmas_give_the_window_box_mean(filter_width, half):
+    """With both sigmas huge every weight is 1: the output is the plain mean
+    of the valid pixels of the (2 floor(fw / 2) + 1)^2 window
+    (src/bilateral_depth_filter.cu:47-54,85), which pins the window and the
+    masking of neighbours."""
+    rng = np.random.default_rng(filter_width)
+    H, W = 14, 17
+    depth = rng.uniform(1.0, 3.0, size=(H, W)).astype(np.float32)
+    mask = rng.uniform(size=(H, W)) < 0.8
+    out = orc.bilateral(depth, mask, filter_width, 1e4, 1e4, 1.0)
+    np.testing.assert_allclose(out, _box_mean(depth, mask, half), rtol=1e-6, atol=0)
+    assert (out[~mask] == 0).all()
+
+
+def _two_pixel_weight(space_sqr, ddepth, sigma_space, sigma_color):
+    return math.exp(-0.5 * (space_sqr / sigma_space ** 2 +
+                            ddepth ** 2 / sigma_color ** 2))
+
+
+@pytest.mark.parametrize("dx,dy", [(2, 1), (-3, 0), (1, -3)])
+def test_bilateral_two_pixel_closed_form(dx, dy):
+    """Two valid pixels a distance (dx, dy) apart with depths d0, d1: each
+    output is (d_self + w d_other) / (1 + w) with
+    w = exp(-0.5 (|dx, dy|^2 / sigma_space^2 + (d1 - d0)^2 / sigma_color^2))
+    (src/bilateral_depth_filter.cu:56-68). The spatial and range terms are
+    of different sizes here, so exchanging the two sigmas, or dropping one
+    term, moves the output by far more than the tolerance."""
+    H, W = 12, 12
+    r0, c0 = 5, 5
+    d0, d1 = 1.0, 1.04
+    depth = np.zeros((H, W), np.float32)
+    mask = np.zeros((H, W), bool)
+    depth[r0, c0], depth[r0 + dy, c0 + dx] = d0, d1
+    mask[r0, c0] = mask[r0 + dy, c0 + dx] = True
+    ss, sc = 2.0, 0.05
+    out = orc.bilateral(depth, mask, 6, sc, ss, 1.0)
+    w = _two_pixel_weight(dx * dx + dy * dy, np.float32(d1) - np.float32(d0), ss, sc)
+    assert 0.01 < w < 0.9  # both terms matter
+    np.testing.assert_allclose(out[r0, c0], (d0 + w * d1) / (1 + w), rtol=2e-6)
+    np.testing.assert_allclose(out[r0 + dy, c0 + dx], (d1 + w * d0) / (1 + w), rtol=2e-6)
+    # The plausible mistakes give other values.
+    for wrong in (_two_pixel_weight(dx * dx + dy * dy, d1 - d0, sc, ss),
+                  _two_pixel_weight(dx * dx + dy * dy, 0.0, ss, sc),
+                  _two_pixel_weight(0.0, d1 - d0, ss, sc)):
+        assert abs((d0 + wrong * d1) / (1 + wrong) - out[r0, c0]) > 1e-4
+    # A window of half width 2 (filter_width 4 or 5) leaves out a neighbour
+    # three pixels away: each pixel keeps its own depth.
+    out4 = orc.bilateral(depth, mask, 4, sc, ss, 1.0)
+    if max(abs(dx), abs(dy)) == 3:
+        assert out4[r0, c0] == np.float32(d0)
+    else:
+        np.testing.assert_allclose(out4[r0, c0], out[r0, c0], rtol=1e-7)
+
+
+def test_bilateral_depth_scale_multiplies_in_and_divides_out():
+    """Millimetre input with depth_scale 0.001: the weights see metres
+    (sigma_color 0.05 m), the output is back in millimetres and, for an
+    integer image, truncated (src/bilateral_depth_filter.cu:38,64,72-73)."""
+    H, W = 9, 9
+    depth = np.zeros((H, W), np.int32)
+    mask = np.zeros((H, W), bool)
+    depth[4, 4], depth[4, 6] = 1000, 1040
+    mask[4, 4] = mask[4, 6] = True
+    out = orc.bilateral(depth, mask, 6, 0.05, 2.0, 0.001)
+    w = _two_pixel_weight(4.0, 0.04, 2.0, 0.05)
+    assert out[4, 4] == int((1.0 + w * 1.04) / (1 + w) * 1000)
+    assert out[4, 6] == int((1.04 + w * 1.0) / (1 + w) * 1000)
+
+
+def test_bilateral_step_edge_does_not_mix():
+    """A depth step far above sigma_color: each side is filtered only with
+    its own side (the range weight underflows to zero across the step), so
+    the output is the space-weighted mean of the same-side pixels."""
+    rng = np.random.default_rng(4)
+    H, W = 12, 16
+    depth = np.where(np.arange(W)[None, :] < 8, 1.0, 2.0).repeat(H, 0)
+    depth = (depth + rng.uniform(-1e-4, 1e-4, size=(H, W))).astype(np.float32)
+    mask = np.ones((H, W), bool)
+    out = orc.bilateral(depth, mask, 6, 0.03, 4.5, 1.0)
+    for r, c in [(5, 7), (5, 8), (0, 7), (11, 8)]:
+        num = den = 0.0
+        for y in range(max(r - 3, 0), min(r + 4, H)):
+            for x in range(max(c - 3, 0), min(c + 4, W)):
+                if (x < 8) != (c < 8):
+                    continue
+                w = _two_pixel_weight((x - c) ** 2 + (y - r) ** 2,
+                                      float(depth[y, x]) - float(depth[r, c]), 4.5, 0.03)
+                num += w * float(depth[y, x])
+                den += w
+        np.testing.assert_allclose(out[r, c], num / den, rtol=1e-6)
+
+
 def test_normals_of_planes_and_isolated_pixels():
     H, W, f = 16, 20, 30.0
     k = np.array([[f, 0, (W - 1) / 2], [0, f, (H - 1) / 2], [0, 0, 1]], np.float32)
@@ -177,3 +285,30 @@ def test_gaussian_blur_of_a_delta_is_the_normalised_kernel():
     assert abs(out.sum() - 1) < 1e-6
     const = np.full((2, 5, 7), 0.3, np.float32)
     np.testing.assert_allclose(orc.gaussian_blur3(const), 0.3, rtol=1e-6)
+
+
+def test_gaussian_blur_border_is_kornias_reflect_default():
+    """feature_pyramid blurs with kornia's GaussianBlur2d((3, 3), (0.849,
+    0.849)) (slamtb/processing.py:148), whose published default border is
+    'reflect' (mirror without repeating the edge texel). kornia is absent, so
+    the oracle is checked against the same operation written with torch:
+    reflect padding and a 2-D convolution with the normalised sampled
+    Gaussian, on random images including every border texel. Replicate and
+    zero padding differ at the borders by far more than the tolerance."""
+    import torch
+    import torch.nn.functional as F
+    rng = np.random.default_rng(12)
+    img = rng.uniform(0, 1, size=(3, 17, 23)).astype(np.float32)
+    sigma = 0.849
+    g = np.exp(-np.arange(-1, 2) ** 2 / (2 * sigma ** 2))
+    g /= g.sum()
+    k2 = torch.from_numpy(np.outer(g, g))[None, None].repeat(3, 1, 1, 1)
+    x = torch.from_numpy(img).double()[None]
+
+    def blur(mode):
+        pad = F.pad(x, (1, 1, 1, 1), mode=mode) if mode else F.pad(x, (1, 1, 1, 1))
+        return F.conv2d(pad, k2, groups=3)[0].numpy()
+    out = orc.gaussian_blur3(img, sigma)
+    np.testing.assert_allclose(out, blur("reflect"), rtol=0, atol=2e-7)
+    assert np.abs(out - blur("replicate")).max() > 1e-2
+    assert np.abs(out - blur(None)).max() > 1e-2
