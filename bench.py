@@ -302,7 +302,7 @@ def registration_section(args, ms, levels, run, top, pair):
     return out
 
 
-def batched_section(seed, dev, npairs, tune):
+def batched_section(seed, dev, npairs, tune, with_oracle=True):
     """npairs pairs drawn from {160x120, 320x240, 640x480, 1280x720}, 3 levels
     each (10/5/3 it, geometric + 3-ch features): one batched hipGraph against
     the same pairs one after another through the single-pair path (also one
@@ -359,7 +359,24 @@ def batched_section(seed, dev, npairs, tune):
         gt_t.append(e[1])
     errs = np.array(errs)
     px_iters = sum(L.n * est.num_iters for levels in pairs for est, L in levels)
-    return {
+    # Algorithmic bytes of the whole batch (every level and iteration of
+    # every pair): 12s+18 (correspondence) + 9s+9+2Cs (Jacobians) per
+    # source pixel and iteration counting all pixels, and the honest count
+    # (mask and map words for every pixel, source data and gathers for
+    # valid pixels, Jacobian gathers for matched pixels; each pair's final
+    # match ratio stands for its matched fraction at every level).
+    s_b, C = 4, 3
+    b_all = b_honest = 0.0
+    for i, levels in enumerate(pairs):
+        mr = res_b[i].match_ratio
+        for est, L in levels:
+            n, it = L.n, est.num_iters
+            valid = float(L.src_mask.float().mean().item())
+            b_all += n * it * (12 * s_b + 18 + 9 * s_b + 9 + 2 * C * s_b)
+            b_honest += n * it * (1 + 8 + valid * (6 * s_b + 1 + 6 * s_b + 8) +
+                                  1 + valid * (3 * s_b + 8) +
+                                  min(mr, valid) * (6 * s_b + 2 * C * s_b))
+    out = {
         "pairs": npairs, "tile_px": batch.plans[0].sizes.tile_px,
         "grid": batch.plans[0].params.grid_update,
         "batched_pairs_per_s": npairs / t_batch,
@@ -367,11 +384,61 @@ def batched_section(seed, dev, npairs, tune):
         "speedup_vs_plain": t_plain / t_batch,
         "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3,
         "batched_gpx_iters_per_s": px_iters / t_batch / 1e9,
-        "max_pose_err_vs_single_pair": [float(np.nanmax(errs[:, 0])),
-                                         float(np.nanmax(errs[:, 1]))],
+        "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                     "bytes_all_pixels": b_all, "bytes_honest": b_honest,
+                     "achieved_all_pixels": b_all / t_batch / 1e9,
+                     "achieved": b_honest / t_batch / 1e9,
+                     "frac": b_honest / t_batch / 1e9 / HBM_PEAK_GBS,
+                     "frac_all_pixels": b_all / t_batch / 1e9 / HBM_PEAK_GBS,
+                     "plain_frac_all_pixels": b_all / t_plain / 1e9 / HBM_PEAK_GBS},
+        "free_run_max_pose_diff_vs_single_pair": [float(np.nanmax(errs[:, 0])),
+                                                  float(np.nanmax(errs[:, 1]))],
         "median_pose_err_vs_gt": [float(np.median(gt_r)), float(np.median(gt_t))],
         "resolutions": {f"{W}x{H}": sizes.count((W, H)) for W, H in choices},
     }
+    if with_oracle:
+        out["pose_err_vs_decision_matched_oracle"] = batched_oracle_check(batch, pairs)
+    return out
+
+
+def batched_oracle_check(batch, pairs):
+    """Each pair's last Gauss-Newton iteration against the oracle run from
+    the same pose (the same FP32 per-pixel decisions, FP64 sums and solve):
+    every level but the last, and the last level but its last iteration,
+    run as in the timed graph; the states are read; the last iteration runs;
+    the oracle takes that one iteration from the read poses. Per pair the
+    (rotation, translation) difference; the maximum over the pairs."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc
+    last = len(batch.plans) - 1
+    for li in range(last + 1):
+        batch.begin_level(li)
+        n = batch.plans[li].est.num_iters
+        batch.iterate_level(li, n - 1 if li == last else n)
+    before = batch.read_states()
+    batch.iterate_level(last, 1)
+    after = batch.read_states()
+    pose = lambda st: np.array(list(st.transform), np.float64).reshape(4, 4)
+    per_pair = []
+    for p, levels in enumerate(pairs):
+        est, L = levels[last]
+        k = np.array([[L.fx, 0, L.cx], [0, L.fy, L.cy], [0, 0, 1]], np.float32)
+        c = lambda t: t.detach().float().cpu().numpy()
+        ref = orc.icp_estimate(
+            k, c(L.src_points), c(L.src_normals), L.src_mask.cpu().numpy(),
+            c(L.tgt_points), c(L.tgt_normals), L.tgt_mask.cpu().numpy(), 1,
+            src_feats=c(L.src_feats), tgt_feats=c(L.tgt_feats),
+            transform=pose(before[p]), geom_weight=est.geom_weight,
+            feat_weight=est.feat_weight, distance_threshold=est.distance_threshold,
+            normals_angle_thresh=est.normals_angle_thresh,
+            feat_residual_thresh=est.feat_residual_thresh, with_information=False)
+        per_pair.append(orc.pose_error(ref["transform"], pose(after[p])))
+    per_pair = np.array(per_pair)
+    return {"max": [float(per_pair[:, 0].max()), float(per_pair[:, 1].max())],
+            "per_pair": per_pair.tolist(),
+            "check": "last iteration of the finest level of every pair, from "
+                     "the GPU's pose before it"}
 
 
 def bf16_section(seed, dev, with_oracle):
@@ -918,7 +985,8 @@ def main():
         out.update(registration_section(args, ms, levels, run, top, pair))
     if args.pairs > 0:
         log(f"batched workload: {args.pairs} pairs")
-        out["batched"] = batched_section(args.seed, dev, args.pairs, not args.no_tune)
+        out["batched"] = batched_section(args.seed, dev, args.pairs, not args.no_tune,
+                                         not args.no_oracle)
     if not args.no_bf16:
         log("1280x720 BF16 / FP32 storage workload")
         out["bf16_1280x720"] = bf16_section(args.seed, dev, not args.no_oracle)

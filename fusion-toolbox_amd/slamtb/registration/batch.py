@@ -74,39 +74,63 @@ class BatchedMultiscaleICP:
     def _args(self, plan):
         return (_lib.ptr(plan.plan), ctypes.byref(plan.sizes))
 
+    def _scratch(self):
+        return _lib.ptr(self.scratch), self.scratch.numel()
+
+    def begin_level(self, i, transforms=None):
+        """Reset every pair for level ``i`` (coarse first): level 0 starts
+        from the staged initial poses when ``transforms`` is given (see
+        ``set_initial``), else identity; later levels start from each pair's
+        last accepted pose of the level before."""
+        scratch, nbytes = self._scratch()
+        t0 = _lib.ptr(self.t0) if (i == 0 and transforms is not None) else None
+        _lib.check(_lib.lib().stb_icp_batch_begin(
+            *self._args(self.plans[i]), _lib.ptr(self.states), t0, int(i > 0),
+            scratch, nbytes, _lib.stream()), "stb_icp_batch_begin")
+
+    def iterate_level(self, i, num_iters=None):
+        """``num_iters`` Gauss-Newton iterations of every pair at level ``i``
+        (default: the level's estimator's count). Iterations enqueued one
+        call at a time give the same states as one call for all of them."""
+        plan = self.plans[i]
+        n = plan.est.num_iters if num_iters is None else int(num_iters)
+        scratch, nbytes = self._scratch()
+        _lib.check(_lib.lib().stb_icp_batch_iterate(
+            *self._args(plan), ctypes.byref(plan.params), _lib.ptr(self.states),
+            scratch, nbytes, n, _lib.stream()), "stb_icp_batch_iterate")
+
+    def information_level(self, i):
+        """Every pair's information matrix at its current pose, level ``i``."""
+        plan = self.plans[i]
+        scratch, nbytes = self._scratch()
+        _lib.check(_lib.lib().stb_icp_batch_information(
+            *self._args(plan), ctypes.byref(plan.params), _lib.ptr(self.states),
+            scratch, nbytes, _lib.stream()), "stb_icp_batch_information")
+
     def enqueue(self, transforms=None):
         """Enqueue every level of every pair (capturable in a hipGraph when
         ``transforms`` is None or already staged with ``set_initial``)."""
-        lib = _lib.lib()
-        s = _lib.stream()
-        scratch, nbytes = _lib.ptr(self.scratch), self.scratch.numel()
-        states = _lib.ptr(self.states)
         for i, plan in enumerate(self.plans):
-            t0 = _lib.ptr(self.t0) if (i == 0 and transforms is not None) else None
-            _lib.check(lib.stb_icp_batch_begin(
-                *self._args(plan), states, t0, int(i > 0), scratch, nbytes, s),
-                "stb_icp_batch_begin")
-            _lib.check(lib.stb_icp_batch_iterate(
-                *self._args(plan), ctypes.byref(plan.params), states, scratch,
-                nbytes, plan.est.num_iters, s), "stb_icp_batch_iterate")
+            self.begin_level(i, transforms)
+            self.iterate_level(i)
             if i == len(self.plans) - 1 and plan.est._geom_step is not None:
-                _lib.check(lib.stb_icp_batch_information(
-                    *self._args(plan), ctypes.byref(plan.params), states,
-                    scratch, nbytes, s), "stb_icp_batch_information")
+                self.information_level(i)
+
+    def read_states(self):
+        """Every pair's device state (one host read for the batch)."""
+        raw = self.states.cpu().numpy()
+        return [_lib.StbIcpState.from_buffer_copy(raw[p].tobytes())
+                for p in range(self.num_pairs)]
 
     def set_initial(self, transforms):
         self.t0.copy_(torch.as_tensor(transforms, dtype=torch.float64).view(-1, 4, 4))
 
     def results(self):
         """Per-pair RegistrationResult (one host read for the batch)."""
-        raw = self.states.cpu().numpy()
         last = self.plans[-1]
-        out = []
-        for p in range(self.num_pairs):
-            st = _lib.StbIcpState.from_buffer_copy(raw[p].tobytes())
-            out.append(result_from_state(st, last.levels[p].n,
-                                         last.est._geom_step is not None))
-        return out
+        return [result_from_state(st, last.levels[p].n,
+                                  last.est._geom_step is not None)
+                for p, st in enumerate(self.read_states())]
 
     def estimate(self, transforms=None):
         if transforms is not None:
