@@ -9,6 +9,8 @@
 // block in all -- 10 us for 512 blocks.)
 #include "gn_device.hpp"
 
+#include <mutex>
+
 namespace stb {
 namespace {
 
@@ -584,6 +586,103 @@ int run_fused(const stb_icp_level* level, const stb_icp_params* params,
 }  // namespace stb
 
 using namespace stb;
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point
+// ---------------------------------------------------------------------------
+//
+// Every GPU the library runs on holds one copy of this array until the
+// process ends. In host code `scratch` is only a stand-in for the symbol:
+// its device address is obtained once per device with hipGetSymbolAddress
+// (with that device current) and only that address is ever passed to
+// kernels.
+constexpr size_t kStaticScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of the current device's copy of `scratch`.
+int static_scratch(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("hipGetDevice failed: %s", hipGetErrorString(e));
+    return STB_ERR_LAUNCH;
+  }
+  STB_REQUIRE(dev >= 0 && dev < kMaxScratchDevices,
+              "device %d is beyond the %d devices the static scratch serves",
+              dev, kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                hipGetErrorString(e));
+      return STB_ERR_LAUNCH;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return STB_OK;
+}
+
+__global__ void copy_state_kernel(const uint64_t* __restrict__ src,
+                                  uint64_t* __restrict__ dst, int words) {
+  for (int i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" size_t stb_icp_noalloc_scratch_bytes(void) {
+  return kStaticScratchBytes;
+}
+
+extern "C" int stb_icp_estimate_noalloc(const stb_icp_level* level,
+                                        const stb_icp_params* params,
+                                        const double* transform0,
+                                        int32_t num_iters,
+                                        int32_t with_information,
+                                        stb_icp_state* out_state,
+                                        void* stream) {
+  const char* what = "stb_icp_estimate_noalloc";
+  STB_REQUIRE(level != nullptr && params != nullptr && out_state != nullptr,
+              "%s: NULL argument", what);
+  STB_REQUIRE(num_iters >= 0, "%s: num_iters must be >= 0", what);
+  STB_REQUIRE(level->height > 0 && level->width > 0, "%s: bad target size %dx%d",
+              what, level->width, level->height);
+  const size_t need = ws_bytes(level->height, level->width);
+  STB_REQUIRE(need <= kStaticScratchBytes,
+              "%s: a %dx%d target needs %zu bytes of scratch, more than the "
+              "%zu bytes this entry point holds (targets up to about 1280x720 "
+              "pixels); use stb_icp_begin / stb_icp_iterate with a workspace",
+              what, level->width, level->height, need, kStaticScratchBytes);
+  void* ws = nullptr;
+  int rc = static_scratch(&ws);
+  if (rc) return rc;
+  rc = stb_icp_begin(ws, kStaticScratchBytes, transform0, 0, level->n_src,
+                     level->height, level->width, stream);
+  if (rc) return rc;
+  if (num_iters > 0) {
+    rc = stb_icp_iterate(level, params, ws, kStaticScratchBytes, num_iters,
+                         stream);
+    if (rc) return rc;
+  }
+  if (with_information && params->use_geom) {
+    rc = stb_icp_information(level, params, ws, kStaticScratchBytes, stream);
+    if (rc) return rc;
+  }
+  static_assert(sizeof(stb_icp_state) % 8 == 0, "state is copied in 8-byte words");
+  hipLaunchKernelGGL(copy_state_kernel, dim3(1), dim3(256), 0, as_stream(stream),
+                     static_cast<const uint64_t*>(ws),
+                     reinterpret_cast<uint64_t*>(out_state),
+                     int(sizeof(stb_icp_state) / 8));
+  return check_launch(what);
+}
 
 extern "C" size_t stb_icp_workspace_bytes(int64_t n_src, int32_t height,
                                           int32_t width) {

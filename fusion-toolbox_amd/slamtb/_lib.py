@@ -150,6 +150,10 @@ _SIGNATURES = {
         _vp, _vp, _vp, _vp, _vp]),
     "stb_rigid_transform_backward": (ctypes.c_int, [
         _i32, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "stb_icp_noalloc_scratch_bytes": (_sz, []),
+    "stb_icp_estimate_noalloc": (ctypes.c_int, [
+        ctypes.POINTER(StbIcpLevel), ctypes.POINTER(StbIcpParams), _vp, _i32,
+        _i32, _vp, _vp]),
     "stb_icp_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "stb_icp_begin": (ctypes.c_int, [_vp, _sz, _vp, _i32, _i64, _i32, _i32, _vp]),
     "stb_icp_iterate": (ctypes.c_int, [

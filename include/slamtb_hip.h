@@ -460,6 +460,29 @@ int stb_icp_information(const stb_icp_level* level,
                         const stb_icp_params* params, void* workspace,
                         size_t workspace_bytes, void* stream);
 
+/* No-allocation variant of stb_icp_begin + stb_icp_iterate (+
+ * stb_icp_information) for one level. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory or pass a workspace
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The workspace is a static device array of
+ * stb_icp_noalloc_scratch_bytes() (16 MiB) that every GPU holds until the
+ * process ends, which bounds the target to about 1280x720 pixels (larger
+ * targets are refused with STB_ERR_INVALID). Resets the pose to
+ * `transform0` (device 4x4 FP64, NULL = identity), runs `num_iters`
+ * iterations with the same kernels and launch shapes as the workspace entry
+ * points (the same results, bit for bit), adds the information matrix when
+ * `with_information` and the geometric term is on, and copies the final
+ * state to `out_state` (device memory, one stb_icp_state). Allocates
+ * nothing. Every call on one GPU shares that GPU's array and each call
+ * starts by clearing what it reads: calls that overlap on one GPU (other
+ * streams, other host threads) race and must be serialised by the caller. */
+size_t stb_icp_noalloc_scratch_bytes(void);
+int stb_icp_estimate_noalloc(const stb_icp_level* level,
+                             const stb_icp_params* params,
+                             const double* transform0, int32_t num_iters,
+                             int32_t with_information, stb_icp_state* out_state,
+                             void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batched Gauss-Newton over many frame pairs of uneven resolution.          */
 /* No reference counterpart: the reference registers one pair per call      */
