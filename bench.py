@@ -491,10 +491,11 @@ def bf16_section(seed, dev, with_oracle):
 
 
 def diff_section(seed, dev, batch=8, steps=5):
-    """5 unrolled GN steps on `batch` pairs at 640x480 (geometric + 3-channel
-    features), forward and the adjoint backward of the squared geodesic pose
-    loss to the ground truth, with respect to the source depth, target points
-    and normals, both feature maps and the initial pose."""
+    """5 unrolled GN steps on `batch` pairs at 640x480 (bilateral-filtered
+    depth, geometric + 3-channel features), forward and the adjoint backward
+    of the squared geodesic pose loss to the ground truth, with respect to
+    the source depth, target points and normals, both feature maps and the
+    initial pose."""
     import numpy as np
     import torch
     from slamtb import synthetic
@@ -507,14 +508,19 @@ def diff_section(seed, dev, batch=8, steps=5):
     pairs = [synthetic.make_pair(seed + 700 + i, 640, 480, kcam=k, channels=3)
              for i in range(batch)]
 
+    from slamtb.processing import bilateral_depth_filter
+
+    def filtered(depth):
+        d = torch.from_numpy(depth).to(dev)
+        return bilateral_depth_filter(d, d > 0, None, 6, 0.03, 4.5, 1.0)
+
     def fpc(depth):
         return FramePointCloud.from_frame(
-            Frame(FrameInfo(kc), depth_image=torch.from_numpy(depth).to(dev)),
-            device=dev)
+            Frame(FrameInfo(kc), depth_image=filtered(depth)), device=dev)
     src = [fpc(p["src_depth"]) for p in pairs]
     tgt = [fpc(p["tgt_depth"]) for p in pairs]
     st = lambda xs: torch.stack(xs).contiguous()
-    depth = st([torch.from_numpy(p["src_depth"]) for p in pairs]).to(dev)
+    depth = st([filtered(p["src_depth"]) for p in pairs])
     src_n, src_m = st([f.normals for f in src]), st([f.mask for f in src])
     tgt_p, tgt_n = st([f.points for f in tgt]), st([f.normals for f in tgt])
     tgt_m = st([f.mask for f in tgt])

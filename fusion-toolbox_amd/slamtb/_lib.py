@@ -184,6 +184,8 @@ _SIGNATURES = {
         _vp, _BATCH, _PARAMS, _vp, _vp, _sz, _vp]),
     "stb_icp_batch_information": (ctypes.c_int, [
         _vp, _BATCH, _PARAMS, _vp, _vp, _sz, _vp]),
+    "stb_icp_batch_system64": (ctypes.c_int, [
+        _vp, _BATCH, _PARAMS, _vp, _vp, _vp, _sz, _vp]),
     "stb_icp_batch_adjoint": (ctypes.c_int, [
         _vp, _BATCH, _PARAMS, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
 }

@@ -2,14 +2,15 @@
 
 The forward pass runs ``num_steps`` fused GN iterations for a batch of frame
 pairs (the batched kernels of libslamtb_hip.so) and records, per step, only
-the pose T_k and the reduced 6x6 system (A_k, b_k). The backward pass walks
-the steps in reverse:
+the pose T_k and whether the step was accepted. The backward pass walks the
+steps in reverse:
 
 * through the pose update T_{k+1} = exp_rt(x_k) T_k: dL/dU_k = G T_k^T,
   dL/dT_k += U_k^T G and dL/dx_k by the exp_rt_to_matrix backward
   (src/exp_rt.hpp:87-141);
 * through the solve x_k = A_k^-1 b_k by its implicit adjoint:
-  lambda = A_k^-1 dL/dx_k;
+  lambda = A_k^-1 dL/dx_k, with (A_k, b_k) recomputed at T_k in FP64 per
+  pixel with the forward's decisions (stb_icp_batch_system64);
 * through the per-pixel Jacobians and residuals, which the adjoint kernel
   recomputes at T_k (stb_icp_batch_adjoint) instead of storing them, into
   source points (hence source depth), target points and normals, source and
@@ -31,6 +32,10 @@ from slamtb import _lib
 from slamtb._cslamtb import ExpRtToMatrixOp
 
 from ._fused import Level, STATE_BYTES, make_params
+
+# Slot of JtJ[a][c] among the 21 upper-triangle slots, for every (a, c).
+_TRI = torch.tensor([min(a, c) * (11 - min(a, c)) // 2 + max(a, c)
+                     for a in range(6) for c in range(6)])
 
 _OFF = {name: getattr(_lib.StbIcpState, name).offset
         for name in ("transform", "last_jtj", "last_jr", "iterations")}
@@ -132,6 +137,17 @@ class _Batch:
             _lib.ptr(self.scratch), self.scratch.numel(), n, _lib.stream()),
             "stb_icp_batch_iterate")
 
+    def system64(self):
+        """Every pair's normal equations at its current pose recomputed in
+        FP64 per pixel with the forward's decisions: A [B, 6, 6], b [B, 6]."""
+        out = torch.empty(self.B, 27, dtype=torch.float64, device=self.states.device)
+        _lib.check(_lib.lib().stb_icp_batch_system64(
+            *self._common(), ctypes.byref(self.params), _lib.ptr(self.states),
+            _lib.ptr(out), _lib.ptr(self.scratch), self.scratch.numel(),
+            _lib.stream()), "stb_icp_batch_system64")
+        A = out[:, _TRI].view(self.B, 6, 6)
+        return A, out[:, 21:27].contiguous()
+
     def adjoint(self, grads_dev, coef, dT):
         _lib.check(_lib.lib().stb_icp_batch_adjoint(
             *self._common(), ctypes.byref(self.params), _lib.ptr(self.states),
@@ -158,13 +174,11 @@ class _UnrolledGN(torch.autograd.Function):
                     tgt_feats if use_feat else None, params, cfg["tile_px"])
         T = exp_rt(x0.double())
         bt.begin(T)
-        Ts, As, bs, acc = [], [], [], []
+        Ts, acc = [], []
         prev = _iterations(bt.states)
         for _ in range(cfg["num_steps"]):
             Ts.append(_state_f64(bt.states, "transform", 16).view(B, 4, 4))
             bt.iterate(1)
-            As.append(_state_f64(bt.states, "last_jtj", 36).view(B, 6, 6))
-            bs.append(_state_f64(bt.states, "last_jr", 6).view(B, 6))
             it = _iterations(bt.states)
             acc.append(it > prev)
             prev = it
@@ -174,7 +188,7 @@ class _UnrolledGN(torch.autograd.Function):
         ctx.kcam = kcam
         ctx.hw = (B, H, W)
         ctx.use_feat = use_feat
-        ctx.steps = (Ts, As, bs, acc)
+        ctx.steps = (Ts, acc)
         ctx.save_for_backward(x0, src_points)
         return T_out
 
@@ -182,7 +196,7 @@ class _UnrolledGN(torch.autograd.Function):
     def backward(ctx, grad_T):
         x0, src_points = ctx.saved_tensors
         bt, (B, H, W) = ctx.bt, ctx.hw
-        Ts, As, bs, acc = ctx.steps
+        Ts, acc = ctx.steps
         dev = src_points.device
         C = bt.levels[0].channels if ctx.use_feat else 0
         n = H * W
@@ -202,12 +216,20 @@ class _UnrolledGN(torch.autograd.Function):
         eye = torch.eye(6, dtype=torch.float64, device=dev)
         for k in reversed(range(len(Ts))):
             ok = acc[k]
-            A = torch.where(ok[:, None, None], As[k], eye)
+            # The step's system recomputed in FP64 per pixel at T_k with the
+            # forward's decisions: the forward's own sums carry the FP32
+            # rounding of u and v (up to 3e-5 px at 640x480, ~1e-4 in the
+            # sampled feature derivatives), and dL/dT_k, the difference of
+            # two nearly cancelling terms, would grow that error about 2.5x
+            # per step.
+            bt.begin(Ts[k])
+            A64, b64 = bt.system64()
+            A = torch.where(ok[:, None, None], A64, eye)
             # A is symmetric positive definite for every accepted step (its
             # device Cholesky succeeded); one factorisation serves both
             # solves, with no host synchronisation.
             Lc = torch.linalg.cholesky_ex(A)[0]
-            x = torch.cholesky_solve(torch.where(ok[:, None], bs[k], 0.0)[..., None],
+            x = torch.cholesky_solve(torch.where(ok[:, None], b64, 0.0)[..., None],
                                      Lc)[..., 0]
             U = exp_rt(x)
             dU = G[:, :3] @ Ts[k].transpose(1, 2)
@@ -216,7 +238,6 @@ class _UnrolledGN(torch.autograd.Function):
             lam = torch.cholesky_solve(xbar[..., None], Lc)[..., 0]
             coef = torch.cat([lam, x], 1)
             coef = torch.where(ok[:, None], coef, 0.0).contiguous()
-            bt.begin(Ts[k])
             bt.adjoint(grads_dev, coef, dT)
             dTk = torch.zeros_like(G)
             dTk[:, :3] = dT.view(B, 3, 4)

@@ -566,6 +566,22 @@ int stb_icp_batch_adjoint(const void* plan, const stb_icp_batch_sizes* sizes,
                           double* dT, void* scratch, size_t scratch_bytes,
                           void* stream);
 
+/* The normal equations of one Gauss-Newton step of every pair at its current
+ * pose (reset with stb_icp_batch_begin), recomputed for the adjoint: the
+ * same correspondences, feature gate and bilinear cells as the forward's
+ * FP32 arithmetic, every smooth quantity (transformed point, u and v,
+ * bilinear fractions, J, r) and every sum in FP64. systems [num_pairs, 27]
+ * holds the weighted JtJ upper triangle (21, row-major) and the weighted Jr
+ * (6), combined as stb_icp_iterate combines them. FP32 storage only. No
+ * reference counterpart: the forward's FP32 per-pixel positions round u
+ * and v by up to 3e-5 px at 640x480, which moves the sampled feature
+ * derivatives by ~1e-4, and the adjoint's solves need the system without
+ * that rounding. */
+int stb_icp_batch_system64(const void* plan, const stb_icp_batch_sizes* sizes,
+                           const stb_icp_params* params, void* states,
+                           double* systems, void* scratch, size_t scratch_bytes,
+                           void* stream);
+
 #ifdef __cplusplus
 }
 #endif
