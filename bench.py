@@ -39,8 +39,13 @@ synchronisations. Also reported:
   * the reference's own RGB-D test frames: the Registration notebook's pair
     against the camera poses recorded with the frames, and frame-to-frame
     tracking over 15 frames;
+  * config 1 (320x240 geometric, one level, 10 iterations): iterations per
+    second, the registration per pair, and its pose, JtJ and Jtr against the
+    oracle, free-running and step by step from the GPU's poses;
   * the CPU baseline: the oracle's OpenMP port of the reference CPU path on
-    this host, at all threads and at one thread.
+    this host, at all threads and at one thread, as medians of 20 runs after
+    3 warm-ups: one 640x480 iteration, the 3-level 640x480 registration and
+    the config-1 registration.
 """
 import argparse
 import json
@@ -238,44 +243,85 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(level, est, seconds):
-    """Reference-structured CPU path (OpenMP per-pixel loops, mutex scatter,
-    materialised partials, FP32 sums, FP64 solve) timed on this host: a
-    bounded sample of 640x480 GN iterations at all threads, then at one."""
+def _median_time(fn, runs=20, warmup=3):
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts)
+
+
+def _oracle_level_args(L):
+    import numpy as np
+    c = lambda t: t.detach().float().cpu().numpy()
+    k = np.array([[L.fx, 0, L.cx], [0, L.fy, L.cy], [0, 0, 1]], np.float32)
+    args = (k, c(L.src_points), c(L.src_normals), L.src_mask.cpu().numpy(),
+            c(L.tgt_points), c(L.tgt_normals), L.tgt_mask.cpu().numpy())
+    feats = (dict(src_feats=c(L.src_feats), tgt_feats=c(L.tgt_feats))
+             if L.has_features else {})
+    return args, feats
+
+
+def _cpu_registration(levels, parallel_kw):
+    """The reference CPU path over a pyramid (coarse first): each level's GN
+    loop from the previous level's pose."""
     import numpy as np
     sys.path.insert(0, str(ROOT))
     from oracle import oracle as orc
-    c = lambda t: t.detach().cpu().numpy()
-    k = np.array([[level.fx, 0, level.cx], [0, level.fy, level.cy], [0, 0, 1]],
-                 np.float32)
-    args = (k, c(level.src_points), c(level.src_normals), c(level.src_mask),
-            c(level.tgt_points), c(level.tgt_normals), c(level.tgt_mask))
-    kw = dict(src_feats=c(level.src_feats), tgt_feats=c(level.tgt_feats),
-              geom_weight=est.geom_weight, feat_weight=est.feat_weight,
-              distance_threshold=est.distance_threshold,
-              normals_angle_thresh=est.normals_angle_thresh,
-              feat_residual_thresh=est.feat_residual_thresh,
-              sum_dtype=np.float32, parallel=True, with_information=False)
+    prepared = [(est, _oracle_level_args(L)) for est, L in levels]
 
-    def run(limit_s, min_iters):
-        orc.icp_estimate(*args, 1, **kw)  # warm-up
-        iters, t0, T = 0, time.perf_counter(), np.eye(4)
-        while True:
-            r = orc.icp_estimate(*args, 1, transform=T, **kw)
+    def run():
+        T = np.eye(4)
+        for est, (args, feats) in prepared:
+            r = orc.icp_estimate(*args, est.num_iters, transform=T,
+                                 geom_weight=est.geom_weight,
+                                 feat_weight=est.feat_weight,
+                                 distance_threshold=est.distance_threshold,
+                                 normals_angle_thresh=est.normals_angle_thresh,
+                                 feat_residual_thresh=est.feat_residual_thresh,
+                                 **feats, **parallel_kw)
             if r["transform"] is not None:
                 T = r["transform"]
-            iters += 1
-            el = time.perf_counter() - t0
-            if el >= limit_s and iters >= min_iters:
-                return iters / el, iters, el
+        return T
+    return run
 
+
+def cpu_baseline(levels, config1_levels):
+    """The reference-structured CPU path (OpenMP per-pixel loops, a mutex
+    scatter into the map, materialised [N,6,6] partials summed by torch on
+    the CPU in FP32, FP64 solve) on this host, all threads and one thread:
+    the median of 20 runs after 3 warm-ups of one 640x480 GN iteration
+    (geometric + 3-channel features), of the whole 3-level 640x480
+    registration (10/5/3 iterations) and of the 320x240 geometric
+    registration (10 iterations)."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc
+    import numpy as np
+    kw = dict(sum_dtype=np.float32, parallel=True, with_information=False)
+    est, top = levels[-1]
+    args, feats = _oracle_level_args(top)
+
+    def one_iter():
+        orc.icp_estimate(*args, 1, geom_weight=est.geom_weight,
+                         feat_weight=est.feat_weight,
+                         distance_threshold=est.distance_threshold,
+                         normals_angle_thresh=est.normals_angle_thresh,
+                         feat_residual_thresh=est.feat_residual_thresh,
+                         **feats, **kw)
+    reg2 = _cpu_registration(levels, kw)
+    reg1 = _cpu_registration(config1_levels, kw)
     threads = orc.num_threads()
-    rate, iters, el = run(seconds, 3)
-    orc.set_num_threads(1)
-    rate1, iters1, el1 = run(seconds / 3, 2)
+    out = {"threads": threads}
+    for name, n in (("all_threads", threads), ("one_thread", 1)):
+        orc.set_num_threads(n)
+        out[name] = {"iter_640x480_s": _median_time(one_iter),
+                     "registration_640x480_3level_s": _median_time(reg2),
+                     "registration_320x240_geom_s": _median_time(reg1)}
     orc.set_num_threads(threads)
-    return dict(rate=rate, iters=iters, el=el, threads=threads, rate1=rate1,
-                iters1=iters1, el1=el1)
+    return out
 
 
 def registration_section(args, ms, levels, run, top, pair):
@@ -300,6 +346,91 @@ def registration_section(args, ms, levels, run, top, pair):
         out["pose_err_vs_oracle_fp64"] = list(
             metrics.pose_error(ref64["transform"], T_gpu))
     return out
+
+
+def config1_section(seed, dev, with_oracle):
+    """Config 1: plain geometric point-to-plane ICP at 320x240 (fx = fy =
+    262.5, cx = 159.5, cy = 119.5), central-difference normals, one level,
+    10 iterations, 0.1 m / 20 deg. GPU iterations per second (hipGraph of
+    400 iterations), the whole registration (begin, 10 iterations,
+    information) per pair, and the pose, JtJ and Jtr against the oracle run
+    on the same inputs (FP32 pixels, FP64 sums), free-running and step by
+    step from the GPU's own poses."""
+    import numpy as np
+    import torch
+    from slamtb import metrics, synthetic
+    from slamtb.camera import KCamera
+    from slamtb.frame import Frame, FrameInfo, FramePointCloud
+    from slamtb.registration import ICPOdometry
+    from slamtb.registration._fused import GNRunner
+    from slamtb.registration.icp import result_from_state
+
+    k = synthetic.intrinsics(320, 240).astype(np.float32)
+    pair = synthetic.make_pair(seed + 300, 320, 240, kcam=k)
+    kc = KCamera(torch.from_numpy(k))
+    fps = [FramePointCloud.from_frame(
+        Frame(FrameInfo(kc), depth_image=torch.from_numpy(pair[f"{side}_depth"]).to(dev)),
+        device=dev) for side in ("src", "tgt")]
+    est = ICPOdometry(10, geom_weight=1.0, feat_weight=0.0, distance_threshold=0.1,
+                      normals_angle_thresh=math.radians(20))
+    L = est.level(kc, fps[0].points, fps[0].normals, fps[0].mask, None,
+                  fps[1].points, fps[1].mask, fps[1].normals, None)
+    params = est.params(False)
+    run = GNRunner(dev)
+    run.reserve(L.n, L.height, L.width)
+    K = 400
+    g = capture(lambda: (run.begin(L), run.iterate(L, params, K)))
+    t_it = time_graph(g, 3) / K
+    g_reg = capture(lambda: est.enqueue(run, L))
+    t_reg = time_graph(g_reg, 50)
+    st = run.read_state()
+    res = result_from_state(st, L.n, True)
+    out = {"workload": "320x240 geometric point-to-plane, 1 level, 10 iterations",
+           "iters_per_s": 1.0 / t_it, "ms_per_iter": t_it * 1e3,
+           "registration_ms_per_pair": t_reg * 1e3,
+           "bytes_per_iter_all_pixels": L.n * 111,
+           "gbs_all_pixels": L.n * 111 / t_it / 1e9,
+           "pose_err_vs_gt": list(metrics.pose_error(pair["T_gt"], res.transform)),
+           "match_ratio": res.match_ratio}
+    if with_oracle:
+        sys.path.insert(0, str(ROOT))
+        from oracle import oracle as orc
+        args, _ = _oracle_level_args(L)
+        kw = dict(geom_weight=1.0, feat_weight=0.0, distance_threshold=0.1,
+                  normals_angle_thresh=math.radians(20))
+        ref = orc.icp_estimate(*args, 10, **kw)
+        out["pose_err_vs_oracle"] = list(orc.pose_error(ref["transform"],
+                                                        res.transform.numpy()))
+        jtj = np.array(list(st.best_jtj)).reshape(6, 6)
+        out["jtj_rel_diff_vs_oracle"] = float(np.linalg.norm(jtj - ref["jtj"]) /
+                                              np.linalg.norm(ref["jtj"]))
+        # Step by step: each iteration from the GPU's pose before it.
+        run.begin(L)
+        worst, jr_worst = [0.0, 0.0], 0.0
+        for _ in range(10):
+            T0 = np.array(list(run.read_state().transform)).reshape(4, 4)
+            run.iterate(L, params, 1)
+            s1 = run.read_state()
+            r1 = orc.icp_estimate(*args, 1, transform=T0, with_information=False, **kw)
+            e = orc.pose_error(r1["transform"], np.array(list(s1.transform)).reshape(4, 4))
+            worst = [max(worst[0], e[0]), max(worst[1], e[1])]
+            jr_ref = _oracle_jr(orc, args, T0, kw)
+            jr_worst = max(jr_worst, float(np.linalg.norm(np.array(list(s1.last_jr)) - jr_ref) /
+                                           np.linalg.norm(jr_ref)))
+        out["max_step_err_vs_oracle_from_gpu_pose"] = worst
+        out["max_jtr_rel_diff_vs_oracle"] = jr_worst
+    return out, [(est, L)]
+
+
+def _oracle_jr(orc, args, T, kw):
+    """The oracle's combined Jr of one iteration at pose T (geometric term)."""
+    import numpy as np
+    k, sp, sn, sm, tp, tn, tm = args
+    rt = T[:3].astype(np.float32)
+    cmap = orc.corresp_map(sp, sn, sm, rt, tp, tn, tm, k, kw["distance_threshold"],
+                           kw["normals_angle_thresh"])
+    _, jr, _, _ = orc.geometric(sp, sm, rt, tp, tn, k, cmap)
+    return jr.sum(0, dtype=np.float64) * kw["geom_weight"]
 
 
 def batched_section(seed, dev, npairs, tune, with_oracle=True):
@@ -987,8 +1118,12 @@ def main():
                       "jacobian_fused_kernel": jac_ms,
                       "corresp_solve_kernel_and_gaps": statistics.median(reruns) - jac_ms},
     }
+    c1_levels = None
     if not args.profile:
         out.update(registration_section(args, ms, levels, run, top, pair))
+        log("config 1: 320x240 geometric, 10 iterations")
+        out["config1_320x240"], c1_levels = config1_section(args.seed, dev,
+                                                            not args.no_oracle)
     if args.pairs > 0:
         log(f"batched workload: {args.pairs} pairs")
         out["batched"] = batched_section(args.seed, dev, args.pairs, not args.no_tune,
@@ -1008,18 +1143,32 @@ def main():
     if not args.no_diff:
         log("differentiable GN: 8 pairs at 640x480, 5 steps, forward + backward")
         out["diff_8x640x480"] = diff_section(args.seed, dev)
-    if not args.no_cpu_baseline:
-        log(f"CPU baseline (~{args.cpu_seconds * 4 / 3:.0f} s)")
-        cb = cpu_baseline(top, est, args.cpu_seconds)
+    if not args.no_cpu_baseline and c1_levels is not None:
+        log("CPU baseline (~40 s)")
+        cb = cpu_baseline(levels, c1_levels)
+        best = max(("all_threads", "one_thread"),
+                   key=lambda k: 1.0 / cb[k]["iter_640x480_s"])
         out["cpu_baseline"] = {
-            "value": cb["rate"], "unit": "GN iters/s", "cores": cb["threads"],
-            "kind": "port",
-            "sample": f"{cb['iters']} GN iterations at 640x480 (geometric + "
-                      f"3-ch feature, FP32) of the oracle's OpenMP port of the "
-                      f"reference CPU path in {cb['el']:.1f} s on {cb['threads']} "
-                      f"threads of {cpu_model()}",
-            "one_thread": {"value": cb["rate1"], "iters": cb["iters1"],
-                           "seconds": cb["el1"]}}
+            "value": 1.0 / cb["all_threads"]["iter_640x480_s"], "unit": "GN iters/s",
+            "cores": cb["threads"], "kind": "port",
+            "sample": f"median of 20 runs after 3 warm-ups of one 640x480 GN "
+                      f"iteration (geometric + 3-ch feature, FP32) of the "
+                      f"oracle's OpenMP port of the reference CPU path on "
+                      f"{cb['threads']} threads of {cpu_model()}; also the "
+                      f"3-level 640x480 and the 320x240 geometric registrations "
+                      f"and one thread",
+            "cpu_model": cpu_model(), "faster_thread_setting": best,
+            "all_threads": cb["all_threads"], "one_thread": cb["one_thread"]}
+        if "config1_320x240" in out:
+            c1 = out["config1_320x240"]
+            c1["cpu_registration_s"] = {k: cb[k]["registration_320x240_geom_s"]
+                                        for k in ("all_threads", "one_thread")}
+            c1["speedup_vs_cpu_best"] = (min(c1["cpu_registration_s"].values()) /
+                                         (c1["registration_ms_per_pair"] * 1e-3))
+        out["registration_speedup_vs_cpu_best"] = (
+            min(cb[k]["registration_640x480_3level_s"]
+                for k in ("all_threads", "one_thread")) /
+            (out["registration_ms_per_pair"] * 1e-3)) if "registration_ms_per_pair" in out else None
     print(json.dumps(out), flush=True)
 
 
