@@ -28,7 +28,8 @@ synchronisations. Also reported:
     sums (decision-matched) and all-FP64 -- and against the ground truth;
   * 64 pairs of uneven resolution in one batched hipGraph against the same
     pairs one after another through the single-pair path;
-  * 1280x720 with 32-channel descriptors in BF16 storage against FP32;
+  * 1280x720 with 32-channel descriptors in BF16 and FP16 storage against
+    FP32;
   * 5 unrolled differentiable GN steps on 8 pairs at 640x480, forward and
     adjoint backward;
   * the differentiable frame matcher and projection kernels at 640x480 and
@@ -574,7 +575,9 @@ def batched_oracle_check(batch, pairs):
 
 def bf16_section(seed, dev, with_oracle):
     """1280x720, 4 levels (10/5/3/2 it), geometric + 32-channel unit
-    descriptors; BF16 storage against FP32 storage of the same scene."""
+    descriptors; BF16 and FP16 storage against FP32 storage of the same
+    scene: iterations per second at the top level, its correspondence and
+    Jacobian launches alone, the whole registration, and pose errors."""
     import torch
     from slamtb import synthetic, metrics
     from slamtb.registration import ICPOdometry, MultiscaleRegistration
@@ -588,8 +591,9 @@ def bf16_section(seed, dev, with_oracle):
     pr = synthetic.make_pair(seed + 500, 1280, 720,
                              kcam=synthetic.intrinsics(1280, 720), channels=32,
                              descriptor=True)
-    out = {}
-    for name, storage in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
+    out, poses = {}, {}
+    for name, storage in (("bf16", torch.bfloat16), ("fp16", torch.float16),
+                          ("fp32", torch.float32)):
         levels = gpu_pair_levels(ms, pr, dev, storage=storage)
         est, top = levels[-1]
         params = est.params(True)
@@ -601,23 +605,32 @@ def bf16_section(seed, dev, with_oracle):
         g = capture(lambda: run.iterate(top, params, K))
         run.begin(top)
         t = time_graph(g, 1) / K
-        s, C = (2 if storage == torch.bfloat16 else 4), 32
-        bytes_it = top.n * (12 * s + 18 + 9 * s + 9 + 2 * C * s)
+        s_b, C = (4 if storage == torch.float32 else 2), 32
+        bytes_it = top.n * (12 * s_b + 18 + 9 * s_b + 9 + 2 * C * s_b)
+        run.begin(top)
+        run.iterate(top, params, 2)
+        corr = launch_ms(lambda: run.correspond(top, params), 50)
+        run.linearize(top, params)
+        jac = launch_ms(lambda: run.jacobians(top, params), 50)
+        matched = run.read_state().last_geom_count
         g_reg = capture(lambda: ms.enqueue_levels(run, levels))
         t_reg = time_graph(g_reg, 3)
         res = result_from_state(run.read_state(), top.n, True)
         entry = {"iters_per_s": 1.0 / t, "ms_per_iter": t * 1e3,
                  "gbs_all_pixels": bytes_it / t / 1e9,
+                 "launch_ms": {"correspondence": corr, "jacobian": jac},
+                 "matched_top_level": matched,
                  "registration_ms": t_reg * 1e3,
                  "pose_err_vs_gt": list(metrics.pose_error(pr["T_gt"], res.transform))}
-        if with_oracle and name == "bf16":
+        if with_oracle and name != "fp32":
             ref = oracle_levels([(e, L) for e, L in levels])
             entry["pose_err_vs_oracle"] = list(metrics.pose_error(ref["transform"],
                                                                   res.transform))
         out[name] = entry
-        out[name + "_transform"] = res.transform
-    out["pose_diff_bf16_vs_fp32"] = list(metrics.pose_error(out.pop("fp32_transform"),
-                                                            out.pop("bf16_transform")))
+        poses[name] = res.transform
+    for name in ("bf16", "fp16"):
+        out[f"pose_diff_{name}_vs_fp32"] = list(metrics.pose_error(poses["fp32"],
+                                                                   poses[name]))
     return out
 
 

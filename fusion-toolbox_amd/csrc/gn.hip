@@ -157,11 +157,11 @@ __global__ void __launch_bounds__(kSolveBlock)
   }
   unsigned long long* __restrict__ map = use.sel ? map1 : map0;
   if (idx < hi)
-    corresp_pixel_at(L, R, r.m, r.nm, idx, vs, ps, ns, map, use.tag, dthr2,
-                     cstar);
+    corresp_pixel_store(L, R, r.m, r.nm, idx, vs, ps, ns, map, use.tag, dthr2,
+                        cstar);
   if (TWO_PX && idx1 < hi)
-    corresp_pixel_at(L, R, r.m, r.nm, idx1, vs1, ps1, ns1, map, use.tag, dthr2,
-                     cstar);
+    corresp_pixel_store(L, R, r.m, r.nm, idx1, vs1, ps1, ns1, map, use.tag,
+                        dthr2, cstar);
   for (int64_t i = TWO_PX ? idx1 + kSolveBlock : idx1; i < hi; i += kSolveBlock)
     corresp_pixel(L, R, r.m, r.nm, i, map, use.tag, dthr2, cstar);
 }

@@ -14,6 +14,8 @@
 
 #include "icp_math.hpp"
 
+#include <type_traits>
+
 namespace stb {
 
 constexpr int kSlots = 32;  // 21 JtJ + 6 Jr + r2_geom, r2_feat, n_geom, n_feat, 0
@@ -213,6 +215,74 @@ __device__ inline void corresp_pixel_at(const LevelView<S>& L,
             (unsigned long long)make_key(sqrtf(sq), uint32_t(idx), tag));
 }
 
+// corresp_pixel_at for 16-bit storage. Coordinates rounded to 16 bits put
+// runs of neighbouring source pixels on one target pixel (BF16 keeps 8
+// mantissa bits: ~1.6 cm at 2 m), so lanes of a wavefront would issue
+// atomics to one address, which the memory pipeline serialises. Here the
+// lanes first take the minimum key over each run of consecutive active
+// lanes with the same target pixel (a segmented inclusive scan, six
+// steps), and only the last lane of each run issues the atomic: the same
+// minimum reaches the map. Every lane that reaches the call takes part;
+// lanes outside the exec mask are never read.
+template <typename S>
+__device__ inline void corresp_pixel_at_runs(
+    const LevelView<S>& L, const LevelRes& R, const float* m, const float* nm,
+    int64_t idx, uint8_t valid_src, V3<float> ps, V3<float> ns,
+    unsigned long long* __restrict__ map, uint32_t tag, float dthr2,
+    float cstar) {
+#pragma clang fp contract(off)
+  const V3<float> p = rigid_point(m, ps);
+  const V3<float> nn = rigid_normal(nm, ns);
+  float u, v;
+  project(L.fx, L.fy, L.cx, L.cy, p, u, v);
+  int ui, vi;
+  const bool in_u = round_in_range(u, L.W, ui);
+  const bool in_v = round_in_range(v, L.H, vi);
+  const bool inside = valid_src && in_u && in_v;
+  const int pos = inside ? vi * L.W + ui : 0;
+  const uint8_t valid = res_u8(R.tm, pos);
+  const V3<float> q = res_ld3<S>(R.tp, pos);
+  const V3<float> qn = res_ld3<S>(R.tn, pos);
+  const V3<float> d = sub3(q, p);
+  const float sq = dot3(d, d);
+  const bool keep = inside && (valid != 0) && !(sq > dthr2) &&
+                    !angle_rejects(gate_cos(nn, qn), cstar);
+  const int lane = threadIdx.x & 63;
+  // A lane that keeps nothing gets a target of its own (-1 - lane), so it
+  // joins no run.
+  const int tgt = keep ? pos : -1 - lane;
+  unsigned long long key =
+      keep ? (unsigned long long)make_key(sqrtf(sq), uint32_t(idx), tag)
+           : ~0ull;
+  const unsigned long long exec = __builtin_amdgcn_read_exec();
+#pragma unroll
+  for (int dl = 1; dl < 64; dl <<= 1) {
+    const unsigned long long ok = lane >= dl ? (exec >> (lane - dl)) & 1ull : 0ull;
+    const unsigned long long k2 = __shfl_up(key, dl, 64);
+    const int t2 = __shfl_up(tgt, dl, 64);
+    if (ok && t2 == tgt && k2 < key) key = k2;
+  }
+  const int tn = __shfl_down(tgt, 1, 64);
+  const bool next_ok = lane < 63 && ((exec >> (lane + 1)) & 1ull);
+  const bool last = !(next_ok && tn == tgt);
+  if (keep && last) atomicMin(map + pos, key);
+}
+
+// The per-storage choice: runs merged in the wavefront for 16-bit storage.
+template <typename S>
+__device__ inline void corresp_pixel_store(
+    const LevelView<S>& L, const LevelRes& R, const float* m, const float* nm,
+    int64_t idx, uint8_t valid_src, V3<float> ps, V3<float> ns,
+    unsigned long long* __restrict__ map, uint32_t tag, float dthr2,
+    float cstar) {
+  if constexpr (sizeof(S) == 2)
+    corresp_pixel_at_runs(L, R, m, nm, idx, valid_src, ps, ns, map, tag, dthr2,
+                          cstar);
+  else
+    corresp_pixel_at(L, R, m, nm, idx, valid_src, ps, ns, map, tag, dthr2,
+                     cstar);
+}
+
 template <typename S>
 __device__ inline void corresp_pixel(const LevelView<S>& L, const LevelRes& R,
                                      const float* m, const float* nm,
@@ -222,7 +292,8 @@ __device__ inline void corresp_pixel(const LevelView<S>& L, const LevelRes& R,
   const uint8_t valid_src = res_u8(R.sm, int(idx));
   const V3<float> ps = res_ld3<S>(R.sp, int(idx));
   const V3<float> ns = res_ld3<S>(R.sn, int(idx));
-  corresp_pixel_at(L, R, m, nm, idx, valid_src, ps, ns, map, tag, dthr2, cstar);
+  corresp_pixel_store(L, R, m, nm, idx, valid_src, ps, ns, map, tag, dthr2,
+                      cstar);
 }
 
 // Adds one weighted term J' = w J, r' (already scaled) to the JtJ/Jr slots.
@@ -265,6 +336,70 @@ __device__ inline void res_taps(__amdgpu_buffer_rsrc_t r, int W,
   t[3] = res_ld<S>(r, d + es, 0);
 }
 
+// 16-bit storage: the low and high halves of a 32-bit word widened to FP32.
+template <typename S>
+__device__ inline float widen_lo(uint32_t x);
+template <typename S>
+__device__ inline float widen_hi(uint32_t x);
+template <>
+__device__ inline float widen_lo<__hip_bfloat16>(uint32_t x) {
+  return __uint_as_float(x << 16);
+}
+template <>
+__device__ inline float widen_hi<__hip_bfloat16>(uint32_t x) {
+  return __uint_as_float(x & 0xffff0000u);
+}
+template <>
+__device__ inline float widen_lo<__half>(uint32_t x) {
+  return float(__builtin_bit_cast(_Float16, uint16_t(x)));
+}
+template <>
+__device__ inline float widen_hi<__half>(uint32_t x) {
+  return float(__builtin_bit_cast(_Float16, uint16_t(x >> 16)));
+}
+
+// Two horizontally adjacent 16-bit texels at byte offset o from one 8-byte
+// load at the 4-byte aligned offset at or below o, clamped to `last8` (the
+// channel's bytes - 8, 4-aligned) so that the load ends inside the channel:
+// one load instruction instead of two. A texel pair past the channel's end
+// (the row below the last) reads shifted data that bil_mix discards.
+template <typename S>
+__device__ inline void res_pair16(__amdgpu_buffer_rsrc_t r, int o, int last8,
+                                  float& a, float& b) {
+  const int base = min(o & ~3, last8);
+  const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, base, 0, 0);
+  const int sh = min(o - base, 6) * 8;
+  const uint32_t lo =
+      uint32_t(((uint64_t(w[1]) << 32) | uint64_t(w[0])) >> sh);
+  a = widen_lo<S>(lo);
+  b = widen_hi<S>(lo);
+}
+
+// res_taps for 16-bit storage with the row pairs loaded together; the
+// channel must hold an even number of texels (4-byte aligned size), at
+// least four.
+template <typename S>
+__device__ inline void res_taps_pair(__amdgpu_buffer_rsrc_t r, int W, int last8,
+                                     const Bil<float>& b, float t[4]) {
+  const int o = (b.vi * W + b.ui) * 2;
+  res_pair16<S>(r, o, last8, t[0], t[1]);
+  res_pair16<S>(r, o + W * 2, last8, t[2], t[3]);
+}
+
+// The taps through whichever form the storage and the channel size allow
+// (`pairs` is uniform across the launch).
+template <typename S>
+__device__ inline void res_taps_any(__amdgpu_buffer_rsrc_t r, int W, bool pairs,
+                                    int last8, const Bil<float>& b, float t[4]) {
+  if constexpr (std::is_same<S, __hip_bfloat16>::value) {
+    if (pairs) {
+      res_taps_pair<S>(r, W, last8, b, t);
+      return;
+    }
+  }
+  res_taps<S>(r, W, b, t);
+}
+
 // feature_terms<float, S, S, true, NB> (icp_math.hpp) with its taps and
 // source values read through buffer resources: the same values in the same
 // arithmetic, so the same bits.
@@ -289,12 +424,24 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
   const Bil<float> bv1 = make_bil(u, fmax(v - h, 0.0f));
   // The last channel's five samples first (see feature_terms).
   const __amdgpu_buffer_rsrc_t rl = make_res(L.tf + int64_t(C - 1) * L.HW, L.HW);
+  // BF16 texels load in row pairs when the channel's size is a multiple of
+  // 4 bytes (an even number of texels). BF16 coordinates (8 mantissa bits)
+  // scatter the projections of neighbouring pixels by a pixel or two, so a
+  // wavefront's taps touch more cache lines per load instruction and the
+  // loads are bound by the texture addresser; halving the instructions
+  // pays for the extraction (1280x720, 32 channels: 96.5 to 76.9 us per
+  // Jacobian launch). FP16 keeps one load per texel: its projections are
+  // not scattered, the loads are not the bound, and the extraction cost
+  // more than it saved (60.8 to 74.2 us).
+  const bool pairs = std::is_same<S, __hip_bfloat16>::value && (L.HW & 1) == 0 &&
+                     L.HW >= 4;
+  const int last8 = int(L.HW) * 2 - 8;
   float t0[4], t1[4], t2[4], t3[4], t4[4];
-  res_taps<S>(rl, W, c0, t0);
-  res_taps<S>(rl, W, bu0, t1);
-  res_taps<S>(rl, W, bu1, t2);
-  res_taps<S>(rl, W, bv0, t3);
-  res_taps<S>(rl, W, bv1, t4);
+  res_taps_any<S>(rl, W, pairs, last8, c0, t0);
+  res_taps_any<S>(rl, W, pairs, last8, bu0, t1);
+  res_taps_any<S>(rl, W, pairs, last8, bu1, t2);
+  res_taps_any<S>(rl, W, pairs, last8, bv0, t3);
+  res_taps_any<S>(rl, W, pairs, last8, bv1, t4);
   const float flast = res_ld<S>(rs, so + (C - 1) * sch, 0);
   const float last = bil_mix(H, W, c0, t0);
   const float su0 = bil_mix(H, W, bu0, t1), su1 = bil_mix(H, W, bu1, t2);
@@ -306,7 +453,8 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
       const int c = c0i + k < last_c ? c0i + k : last_c;
-      res_taps<S>(make_res(L.tf + int64_t(c) * L.HW, L.HW), W, c0, taps[k]);
+      res_taps_any<S>(make_res(L.tf + int64_t(c) * L.HW, L.HW), W, pairs, last8,
+                      c0, taps[k]);
       fs[k] = res_ld<S>(rs, so + c * sch, 0);
     }
 #pragma unroll
