@@ -4,8 +4,8 @@
 //   * per-pixel correspondence, Jacobian and information terms over a level
 //     descriptor (any map storage type, FP32 math);
 //   * the slot reduction: 21 unique JtJ + 6 Jr + r2/count bookkeeping per
-//     lane in FP32, recursive-halving across the wavefront (32 exchanges for
-//     32 slots), FP64 across waves through LDS, one FP64 partial per block,
+//     lane in FP32, recursive halving across the wavefront (permlane swaps
+//     and DPP), FP64 across waves through LDS, one FP64 partial per block,
 //     and a fixed-order FP64 sum of the partials in a following launch (by
 //     one block, or by every block of a loop's next correspondence launch);
 //   * the per-pair state update: FP64 Cholesky solve and exp_rt_to_matrix,
@@ -105,8 +105,8 @@ __device__ inline LevelView<S> view_of(const stb_icp_level& L) {
 // so every load here passes 0 there and puts the whole offset in the vector
 // operand. The bilinear taps therefore need no clamping: each target channel
 // has a resource of its own, a tap past the right edge reads the next row's
-// first texel and one past the bottom row returns 0, and bil_mix replaces
-// both by zero exactly as it replaces the clamped re-reads of bil_taps.
+// first texel (which bil_mix_res replaces by zero, as bil_mix replaces the
+// clamped re-reads of bil_taps) and one past the bottom row returns 0.
 // Every array must hold fewer than 2^31 bytes (checked on the host).
 constexpr int kRawBufferWord3 = 0x00020000;  // raw 32-bit data format, CDNA
 
@@ -296,7 +296,16 @@ __device__ inline void corresp_pixel(const LevelView<S>& L, const LevelRes& R,
                       cstar);
 }
 
-// Adds one weighted term J' = w J, r' (already scaled) to the JtJ/Jr slots.
+// acc = x * y + acc, or acc = x * y when the term is the first into the slot
+// (the same value; only a zero's sign can differ from x * y + 0).
+template <bool BORN>
+__device__ inline void acc_fma(float& acc, float x, float y) {
+  acc = BORN ? x * y : fmaf(x, y, acc);
+}
+
+// Adds one weighted term J' = w J, r' (already scaled) to the JtJ/Jr slots
+// (with BORN, assigns it).
+template <bool BORN = false>
 __device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
                                 float r) {
   float jw[6];
@@ -307,21 +316,13 @@ __device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
   for (int a = 0; a < 6; ++a)
 #pragma unroll
     for (int b = a; b < 6; ++b) {
-      acc[k] = fmaf(jw[a], jw[b], acc[k]);
+      acc_fma<BORN>(acc[k], jw[a], jw[b]);
       ++k;
     }
 #pragma unroll
-  for (int a = 0; a < 6; ++a) acc[21 + a] = fmaf(jw[a], r, acc[21 + a]);
+  for (int a = 0; a < 6; ++a) acc_fma<BORN>(acc[21 + a], jw[a], r);
 }
 
-// Point-to-plane and feature terms of source pixel idx when the map names it
-// as its target pixel's owner (src/icp_jacobian_geometric.cu:54-85,
-// src/icp_jacobian_feature.cu:58-109). The owner word, the target point and
-// normal and the feature taps around (u, v) are loaded whatever the owner
-// test says and the terms are accumulated through selects, so all of those
-// loads issue in one batch: a pixel costs two memory round trips. A pixel
-// that does not own its target adds exact zeros (selected, never multiplied
-// in, so NaN or inf in its unused data cannot leak into the sums).
 // The four bilinear taps of one sample from the resource of one [H, W]
 // channel.
 template <typename S>
@@ -400,6 +401,22 @@ __device__ inline void res_taps_any(__amdgpu_buffer_rsrc_t r, int W, bool pairs,
   res_taps<S>(r, W, b, t);
 }
 
+// bil_mix for taps read through a resource over one channel: the taps of
+// the row below the last lie past the resource's end and read 0, which is
+// the value bil_mix selects for them, so only the right-edge select remains
+// (whose tap, the next row's first texel, is inside the channel).
+__device__ inline float bil_mix_res(int W, const Bil<float>& b, const float t[4]) {
+#pragma clang fp contract(off)
+  const bool rx = b.ui + 1 < W;
+  const float v00 = t[0];
+  const float v10 = rx ? t[1] : 0.0f;
+  const float v01 = t[2];
+  const float v11 = rx ? t[3] : 0.0f;
+  const float a = v00 * (1.0f - b.ur) + v10 * b.ur;
+  const float c = v01 * (1.0f - b.ur) + v11 * b.ur;
+  return a * (1.0f - b.vr) + c * b.vr;
+}
+
 // feature_terms<float, S, S, true, NB> (icp_math.hpp) with its taps and
 // source values read through buffer resources: the same values in the same
 // arithmetic, so the same bits.
@@ -443,9 +460,15 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
   res_taps_any<S>(rl, W, pairs, last8, bv0, t3);
   res_taps_any<S>(rl, W, pairs, last8, bv1, t4);
   const float flast = res_ld<S>(rs, so + (C - 1) * sch, 0);
-  const float last = bil_mix(H, W, c0, t0);
-  const float su0 = bil_mix(H, W, bu0, t1), su1 = bil_mix(H, W, bu1, t2);
-  const float sv0 = bil_mix(H, W, bv0, t3), sv1 = bil_mix(H, W, bv1, t4);
+  // Row pairs past the channel's end read shifted data, not 0: BF16 keeps
+  // both selects.
+  auto mix = [&](const Bil<float>& b, const float t[4]) {
+    if constexpr (std::is_same<S, __hip_bfloat16>::value) return bil_mix(H, W, b, t);
+    else return bil_mix_res(W, b, t);
+  };
+  const float last = mix(c0, t0);
+  const float su0 = mix(bu0, t1), su1 = mix(bu1, t2);
+  const float sv0 = mix(bv0, t3), sv1 = mix(bv1, t4);
   float sq = 0.0f;
   const int last_c = C - 2 > 0 ? C - 2 : 0;
   for (int c0i = 0; c0i < C - 1; c0i += NB) {
@@ -459,7 +482,7 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
     }
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-      const float res = bil_mix(H, W, c0, taps[k]) - fs[k];
+      const float res = mix(c0, taps[k]) - fs[k];
       sq += (c0i + k < C - 1) ? res * res : 0.0f;
     }
   }
@@ -485,11 +508,26 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
   return sq;
 }
 
-template <typename S, bool GEOM, bool FEAT, int NB = 4>
+// Point-to-plane and feature terms of source pixel idx when the map names it
+// as its target pixel's owner (src/icp_jacobian_geometric.cu:54-85,
+// src/icp_jacobian_feature.cu:58-109). The owner word, the target point and
+// normal and the feature taps around (u, v) are loaded whatever the owner
+// test says and the terms are accumulated through selects, so all of those
+// loads issue in one batch: a pixel costs two memory round trips. A pixel
+// that does not own its target adds exact zeros (selected, never multiplied
+// in, so NaN or inf in its unused data cannot leak into the sums).
+// No branch depends on one lane's validity (`live`: the pixel exists; its
+// source mask; its projection inside the image): an invalid lane evaluates
+// the terms at the projection of the wavefront's first valid lane, so its
+// loads touch lines that lane loads anyway, and adds zeros; only a
+// wavefront with no valid lane returns early. Without per-lane branches the
+// slots stay in place across the pixel, and with BORN (a thread's first
+// pixel) the terms assign the slots instead of adding to zeros.
+template <typename S, bool GEOM, bool FEAT, int NB = 4, bool BORN = false>
 __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
                                          const LevelRes& R, const float* m,
                                          const unsigned long long* __restrict__ map,
-                                         uint32_t tag, int64_t idx,
+                                         uint32_t tag, int64_t idx, bool live,
                                          uint8_t valid_src, V3<float> ps,
                                          const TermWeights& tw,
                                          float (&acc)[kSlots]) {
@@ -497,15 +535,35 @@ __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
   const V3<float> p = rigid_point(m, ps);
   float u, v;
   project(L.fx, L.fy, L.cx, L.cy, p, u, v);
-  int ui, vi;
+  int ui = 0, vi = 0;
   const bool in_u = round_in_range(u, L.W, ui);
   const bool in_v = round_in_range(v, L.H, vi);
-  if (!valid_src || !in_u || !in_v) return;
+  const bool ok = live && valid_src && in_u && in_v;
+  const uint64_t okm = __ballot(ok);
+  if (okm == 0) {
+    if (BORN) {
+#pragma unroll
+      for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
+    }
+    return;
+  }
+  const int first = __builtin_ctzll(okm);
+  const float uf = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u), first));
+  const float vf = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), first));
+  const int uif = __builtin_amdgcn_readlane(ui, first);
+  const int vif = __builtin_amdgcn_readlane(vi, first);
+  u = ok ? u : uf;
+  v = ok ? v : vf;
+  ui = ok ? ui : uif;
+  vi = ok ? vi : vif;
   const int pos = vi * L.W + ui;
   const __amdgpu_buffer_rsrc_t rmap = make_res(map, L.HW);
   const uint32_t klo = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos, 0, 0);
   const uint32_t khi = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos + 4, 0, 0);
-  const bool own = key_owned((uint64_t(khi) << 32) | klo, uint32_t(idx), tag);
+  // `&`, not `&&`: a short-circuit would put the map load under a branch that
+  // waits for it before the target loads issue.
+  const bool own =
+      ok & key_owned((uint64_t(khi) << 32) | klo, uint32_t(idx), tag);
   if (GEOM) {
     const V3<float> q = res_ld3<S>(R.tp, pos);
     const V3<float> qn = res_ld3<S>(R.tn, pos);
@@ -514,35 +572,42 @@ __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
     const float j[6] = {own ? qn.x : 0.0f, own ? qn.y : 0.0f, own ? qn.z : 0.0f,
                         own ? c.x : 0.0f,  own ? c.y : 0.0f,  own ? c.z : 0.0f};
     const float rs = own ? r : 0.0f;
-    acc_term(acc, j, tw.wg, rs * tw.sg);
-    acc[27] = fmaf(rs, rs, acc[27]);
-    acc[29] += own ? 1.0f : 0.0f;
+    acc_term<BORN>(acc, j, tw.wg, rs * tw.sg);
+    acc_fma<BORN>(acc[27], rs, rs);
+    acc[29] = (BORN ? 0.0f : acc[29]) + (own ? 1.0f : 0.0f);
   }
   if (FEAT) {
+    constexpr bool kBornF = BORN && !GEOM;
     float jf[6], rl;
     const float sq = feature_terms_res<S, NB>(L, idx, p, u, v, jf, rl);
     const bool use = own && !(sq > tw.fthr2);
     float j[6];
 #pragma unroll
     for (int a = 0; a < 6; ++a) j[a] = use ? jf[a] : 0.0f;
-    acc_term(acc, j, tw.wf, (use ? rl : 0.0f) * tw.sf);
-    acc[28] += use ? sq : 0.0f;
-    acc[30] += use ? 1.0f : 0.0f;
+    acc_term<kBornF>(acc, j, tw.wf, (use ? rl : 0.0f) * tw.sf);
+    acc[28] = (kBornF ? 0.0f : acc[28]) + (use ? sq : 0.0f);
+    acc[30] = (kBornF ? 0.0f : acc[30]) + (use ? 1.0f : 0.0f);
+  }
+  if (BORN) {
+    if (!GEOM) acc[27] = acc[29] = 0.0f;
+    if (!FEAT) acc[28] = acc[30] = 0.0f;
+    acc[31] = 0.0f;
   }
 }
 
-// jacobian_pixel_at with the source pixel's mask and point loaded here.
-template <typename S, bool GEOM, bool FEAT, int NB = 4>
+// jacobian_pixel_at with the source pixel's mask and point loaded here
+// (past the arrays' ends the buffer loads return 0).
+template <typename S, bool GEOM, bool FEAT, int NB = 4, bool BORN = false>
 __device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
                                       const float* m,
                                       const unsigned long long* __restrict__ map,
                                       uint32_t tag, int64_t idx,
                                       const TermWeights& tw,
-                                      float (&acc)[kSlots]) {
+                                      float (&acc)[kSlots], bool live = true) {
   const uint8_t valid_src = res_u8(R.sm, int(idx));
   const V3<float> ps = res_ld3<S>(R.sp, int(idx));
-  jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, valid_src, ps,
-                                       tw, acc);
+  jacobian_pixel_at<S, GEOM, FEAT, NB, BORN>(L, R, m, map, tag, idx, live,
+                                             valid_src, ps, tw, acc);
 }
 
 // G^T G of source pixel idx from its matched target point
@@ -579,33 +644,66 @@ __device__ inline void information_pixel(const LevelView<S>& L, const float* m,
 // Reductions
 // ---------------------------------------------------------------------------
 
-// One recursive-halving step: lanes with bit `S` set keep the upper half of
-// their H live slots, the others the lower half, and add the partner's copy.
-template <int S, int H>
-__device__ inline void halve(float (&v)[kSlots], int lane) {
-  const bool up = (lane >> S) & 1;
+// The wavefront sum of 32 slots by recursive halving: each step pairs every
+// lane with a partner that holds the same slots and differs in one lane bit;
+// the lane with the bit set keeps the upper half of the live slots, its
+// partner the lower half, and each adds the other's copy. The two upper
+// bits go through gfx950's permlane swaps, which exchange the halves in one
+// instruction per slot pair with no select; the four lower bits through DPP
+// lane patterns that read the partner inside the add (row_ror:8 pairs lanes
+// i and i ^ 8 of a row, row_half_mirror i and 7 - i of a half-row, the quad
+// permutations i ^ 3 and i ^ 1). The 32 exchanges stay in registers (no
+// ds_bpermute round trips through the LDS unit), and the 24 of the upper
+// steps need no selects.
+constexpr int kDppRowRor8 = 0x128;
+constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppQuad3210 = 0x1B;
+constexpr int kDppQuad1032 = 0xB1;
+
+template <int CTRL>
+__device__ inline float dpp_read(float x) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
+// Lanes 0..31 keep slots k, lanes 32..63 slots k + 16 (k < 16).
+__device__ inline void halve_swap32(float (&v)[kSlots]) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const auto r = __builtin_amdgcn_permlane32_swap(
+        __float_as_uint(v[k]), __float_as_uint(v[k + 16]), false, false);
+    v[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+}
+
+// Rows 0 and 2 keep slots k, rows 1 and 3 slots k + 8 (k < 8).
+__device__ inline void halve_swap16(float (&v)[kSlots]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const auto r = __builtin_amdgcn_permlane16_swap(
+        __float_as_uint(v[k]), __float_as_uint(v[k + 8]), false, false);
+    v[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+}
+
+template <int CTRL, int H>
+__device__ inline void halve_dpp(float (&v)[kSlots], bool up) {
 #pragma unroll
   for (int k = 0; k < H; ++k) {
     const float send = up ? v[k] : v[k + H];
     const float keep = up ? v[k + H] : v[k];
-    v[k] = keep + __shfl_xor(send, 1 << S, 64);
+    v[k] = keep + dpp_read<CTRL>(send);
   }
 }
 
-// After this, lane l < 32 holds the wavefront total of slot bitrev5(l) in v[0]
-// (lanes 32..63 hold the same values).
+// After this, lane l holds the wavefront total of slot l >> 1 in v[0].
 __device__ inline void wave_reduce_slots(float (&v)[kSlots], int lane) {
-  halve<0, 16>(v, lane);
-  halve<1, 8>(v, lane);
-  halve<2, 4>(v, lane);
-  halve<3, 2>(v, lane);
-  halve<4, 1>(v, lane);
-  v[0] += __shfl_xor(v[0], 32, 64);
-}
-
-__device__ inline int bitrev5(int l) {
-  return ((l & 1) << 4) | ((l & 2) << 2) | (l & 4) | ((l & 8) >> 2) |
-         ((l & 16) >> 4);
+  halve_swap32(v);
+  halve_swap16(v);
+  halve_dpp<kDppRowRor8, 4>(v, (lane >> 3) & 1);
+  halve_dpp<kDppRowHalfMirror, 2>(v, (lane >> 2) & 1);
+  halve_dpp<kDppQuad3210, 1>(v, (lane >> 1) & 1);
+  v[0] += dpp_read<kDppQuad1032>(v[0]);
 }
 
 // Reduces the block's slots and stores them in FP64 at `out` (32 doubles),
@@ -618,7 +716,7 @@ __device__ inline void block_partial(float (&acc)[kSlots], double* out,
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   wave_reduce_slots(acc, lane);
-  if (lane < 32) sh[wave * kSlots + bitrev5(lane)] = double(acc[0]);
+  if ((lane & 1) == 0) sh[wave * kSlots + (lane >> 1)] = double(acc[0]);
   __syncthreads();
   if (threadIdx.x < kSlots) {
     double s = 0.0;

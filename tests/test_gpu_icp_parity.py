@@ -18,7 +18,7 @@ from slamtb.registration import ICPOdometry
 from slamtb.registration._fused import Level, make_params, runner_for
 from slamtb.registration.correspondence_map import CorrespondenceMap
 
-from _helpers import prepared_pair, to_dev
+from _helpers import prepared_pair, state_pose, to_dev
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
@@ -337,8 +337,16 @@ def test_map_tags_wrap_around():
     last (255, 254, ...) instead of clearing the map; after tag 0 it sweeps
     the other map and continues there with 255. Across two wraps, with a
     one-off linearisation (which leaves the spare map set) in between, the
-    loop and the split entry points stay bit-identical and the pose matches
-    the oracle's 600-iteration loop."""
+    loop and the split entry points stay bit-identical, and every iteration
+    next to a wrap or to the linearisation takes the oracle's step: the
+    oracle's one iteration from the GPU's pose before it gives the GPU's
+    match count exactly and its pose to 1e-8 (decision-matched; the first
+    step from the 2-degree start is 0.035 rad, FP32 sums leave ~2e-9).
+
+    The free-running 600-iteration poses are compared only loosely: this
+    pair never settles (the oracle's own 600th step is ~2e-5 rad, the loop
+    cycles through correspondence flips), so a 1e-10 difference in one
+    step's FP32 sums can shift the cycle's phase by a step."""
     k = synthetic.intrinsics(160, 120).astype(np.float32)
     P = prepared_pair(3, 160, 120, kcam=k)
     T0 = small_motion(7, 2.0, 0.02)
@@ -356,22 +364,38 @@ def test_map_tags_wrap_around():
     run.iterate(level, params, 300)
     loop = state_bits(run.read_state())
     assert (loop["map_epoch"], loop["map_sel"]) == (255 - (600 - 512), 0)
+    args = (P["kcam"], P["src_pts"], P["src_nrm"], P["src_mask"], P["tgt_pts"],
+            P["tgt_nrm"], P["tgt_mask"])
+    kw = dict(geom_weight=1.0, feat_weight=0.0, distance_threshold=0.1,
+              normals_angle_thresh=ANG20, with_information=False)
+    # Iterations 256 and 512 use tag 0, 257 and 513 the other map's first
+    # tag; the linearisation runs between iterations 300 and 301.
+    checks = (1, 2, 255, 256, 257, 258, 300, 301, 302, 511, 512, 513, 514, 600)
     run.begin(level, torch.from_numpy(T0))
-    for i in range(600):
-        if i == 300:
+    prev = T0
+    worst = 0.0
+    for i in range(1, 601):
+        if i == 301:
             run.linearize(level, params)
         run.correspond(level, params)
         run.update(level, params)
+        if i in checks:
+            st = run.read_state()
+            ref = orc.icp_estimate(*args, 1, transform=prev, **kw)
+            assert ref["match_count"] == st.last_geom_count, i
+            err = orc.pose_error(state_pose(st), ref["transform"])
+            worst = max(worst, *err)
+            assert err[0] < 1e-8 and err[1] < 1e-8, (i, err)
+        if i + 1 in checks:
+            prev = state_pose(run.read_state())
     split = state_bits(run.read_state())
     for key in loop:
         assert loop[key] == split[key], key
     assert loop["iterations"] == 600 and loop["stopped"] == 0
-    ref = orc.icp_estimate(P["kcam"], P["src_pts"], P["src_nrm"], P["src_mask"],
-                           P["tgt_pts"], P["tgt_nrm"], P["tgt_mask"], 600,
-                           geom_weight=1.0, feat_weight=0.0,
-                           distance_threshold=0.1, normals_angle_thresh=ANG20,
-                           transform=T0)
+    ref = orc.icp_estimate(*args, 600, transform=T0, **kw)
     T = np.array(loop["transform"]).reshape(4, 4)
     err = orc.pose_error(T, ref["transform"])
+    print("tag wrap: worst decision-matched step difference", worst,
+          "; free-running 600-iteration difference", err)
     assert ref["iters"] == 600
-    assert err[0] < 1e-6 and err[1] < 1e-6, err
+    assert err[0] < 1e-4 and err[1] < 1e-4, err
