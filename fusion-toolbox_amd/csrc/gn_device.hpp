@@ -9,7 +9,7 @@
 //     and a fixed-order FP64 sum of the partials in a following launch (by
 //     one block, or by every block of a loop's next correspondence launch);
 //   * the per-pair state update: FP64 Cholesky solve and exp_rt_to_matrix,
-//     in one thread or spread over one wavefront.
+//     in one thread or redundantly in every lane of one wavefront.
 #pragma once
 
 #include "icp_math.hpp"
@@ -944,6 +944,73 @@ __device__ inline void store_linearization(stb_icp_state* st,
 // early by the caller so that their latency hides behind the partial sums.
 // Advances the map use (next_map_use) whether or not the update succeeds:
 // the iteration wrote its map either way.
+// The Gauss-Newton update of one summed system (21 JtJ + 6 Jr slots of
+// `tot`): Cholesky of JtJ, L y = Jr, L^T x = y, then Tn = exp(x) T (3x4,
+// row-major). Returns false on a non-positive pivot (Tn is then not set).
+// Straight-line FP64 code with no memory traffic, so it runs at the same
+// speed in one thread or, redundantly, in every lane of a wavefront, and
+// both give the same bits.
+__device__ inline bool gn_update_f64(const double* tot, const double* T,
+                                     double Tn[12]) {
+  double A[21], b[6];
+#pragma unroll
+  for (int k = 0; k < 21; ++k) A[k] = tot[k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) b[k] = tot[21 + k];
+  // L's diagonal enters only through its reciprocals rinv[j] = 1/sqrt(d2):
+  // the hardware reciprocal square root refined by two Newton steps
+  // (relative error ~1e-16, like sqrt followed by a divide, but a
+  // dependent chain of 7 instructions instead of ~25).
+  double L[21], rinv[6];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double d2 = A[tri_up(j, j)];
+#pragma unroll
+    for (int q = 0; q < j; ++q) d2 -= L[tri_lo(j, q)] * L[tri_lo(j, q)];
+    ok = ok && (d2 > 0.0);
+    double r = __builtin_amdgcn_rsq(d2);
+    r = fma(r * 0.5, fma(-d2 * r, r, 1.0), r);
+    r = fma(r * 0.5, fma(-d2 * r, r, 1.0), r);
+    rinv[j] = r;
+#pragma unroll
+    for (int i = j + 1; i < 6; ++i) {
+      double v = A[tri_up(j, i)];
+#pragma unroll
+      for (int q = 0; q < j; ++q) v -= L[tri_lo(i, q)] * L[tri_lo(j, q)];
+      L[tri_lo(i, j)] = v * rinv[j];
+    }
+  }
+  if (!ok) return false;
+  double y[6], x[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {  // L y = b
+    double v = b[i];
+#pragma unroll
+    for (int q = 0; q < i; ++q) v -= L[tri_lo(i, q)] * y[q];
+    y[i] = v * rinv[i];
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {  // L^T x = y, the newest x[q] last
+    double v = y[i];
+#pragma unroll
+    for (int q = 5; q > i; --q) v -= L[tri_lo(q, i)] * x[q];
+    x[i] = v * rinv[i];
+  }
+  double U[12];
+  exp_rt_f64(x, U);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double v = U[i * 4 + 0] * T[j] + U[i * 4 + 1] * T[4 + j] +
+                 U[i * 4 + 2] * T[8 + j];
+      if (j == 3) v += U[i * 4 + 3];
+      Tn[i * 4 + j] = v;
+    }
+  return true;
+}
+
 __device__ inline void gn_finalize(stb_icp_state* st, const double* tot,
                                    double gw, double fw, double* w, int iters,
                                    int sel, uint32_t tag) {
@@ -951,66 +1018,17 @@ __device__ inline void gn_finalize(stb_icp_state* st, const double* tot,
   store_linearization(st, tot);
   __syncthreads();
   if (t == 0) {
-    double A[21], b[6], T[12];
-#pragma unroll
-    for (int k = 0; k < 21; ++k) A[k] = tot[k];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) b[k] = tot[21 + k];
+    double T[12], Tn[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) T[k] = w[k];
-    // L's diagonal enters only through its reciprocals rinv[j] = 1/sqrt(d2):
-    // the hardware reciprocal square root refined by two Newton steps
-    // (relative error ~1e-16, like sqrt followed by a divide, but a
-    // dependent chain of 7 instructions instead of ~25).
-    double L[21], rinv[6];
-    bool ok = true;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      double d2 = A[tri_up(j, j)];
-#pragma unroll
-      for (int q = 0; q < j; ++q) d2 -= L[tri_lo(j, q)] * L[tri_lo(j, q)];
-      ok = ok && (d2 > 0.0);
-      double r = __builtin_amdgcn_rsq(d2);
-      r = fma(r * 0.5, fma(-d2 * r, r, 1.0), r);
-      r = fma(r * 0.5, fma(-d2 * r, r, 1.0), r);
-      rinv[j] = r;
-#pragma unroll
-      for (int i = j + 1; i < 6; ++i) {
-        double v = A[tri_up(j, i)];
-#pragma unroll
-        for (int q = 0; q < j; ++q) v -= L[tri_lo(i, q)] * L[tri_lo(j, q)];
-        L[tri_lo(i, j)] = v * rinv[j];
-      }
-    }
+    const bool ok = gn_update_f64(tot, T, Tn);
     if (ok) {
-      double y[6], x[6];
-#pragma unroll
-      for (int i = 0; i < 6; ++i) {  // L y = b
-        double v = b[i];
-#pragma unroll
-        for (int q = 0; q < i; ++q) v -= L[tri_lo(i, q)] * y[q];
-        y[i] = v * rinv[i];
-      }
-#pragma unroll
-      for (int i = 5; i >= 0; --i) {  // L^T x = y, the newest x[q] last
-        double v = y[i];
-#pragma unroll
-        for (int q = 5; q > i; --q) v -= L[tri_lo(q, i)] * x[q];
-        x[i] = v * rinv[i];
-      }
-      double U[12];
-      exp_rt_f64(x, U);
       Rigid<float> r;
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          double v = U[i * 4 + 0] * T[j] + U[i * 4 + 1] * T[4 + j] +
-                     U[i * 4 + 2] * T[8 + j];
-          if (j == 3) v += U[i * 4 + 3];
-          w[12 + i * 4 + j] = v;
-          r.m[i * 4 + j] = float(v);
-        }
+      for (int k = 0; k < 12; ++k) {
+        w[12 + k] = Tn[k];
+        r.m[k] = float(Tn[k]);
+      }
       make_normal_matrix(r);
 #pragma unroll
       for (int i = 0; i < 12; ++i) st->rt[i] = r.m[i];
@@ -1103,75 +1121,26 @@ struct SolveRecord {
   int32_t pad;
 };
 
-__device__ inline double readlane_f64(double v, int lane) {
-  const long long b = __double_as_longlong(v);
-  const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(b), lane));
-  const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(b >> 32), lane));
-  return __longlong_as_double(
-      (long long)((static_cast<unsigned long long>(hi) << 32) | lo));
-}
-
-// Cholesky solve of the system in `tot` and the pose update, spread over the
-// 64 lanes of one wavefront (every lane calls it). Lane 6i+k (k < 6) holds
-// element (i, k) of the trailing matrix and lane 36+i element i of the
-// right-hand side; eliminating column j, every lane (i, k) with i, k > j
-// subtracts L[i][j] L[k][j], and lane 36+i (i > j) subtracts L[i][j] y[j].
-// Those are the subtractions of gn_finalize's one-thread left-looking
-// factorisation and forward substitution, in the same order and on the same
-// operands, so both give the same bits -- and the forward substitution
-// rides along with the factorisation instead of following it. The back
-// substitution and exp_rt run on wave-uniform values as in gn_finalize.
-// `To` (LDS) holds rows 0..2 of the current pose; lanes 0..11 leave the
-// updated rows in `Tn` and their float values in `rtf`. Returns whether every
-// pivot was positive.
+// The update of a loop iteration from the summed system in LDS, by one
+// whole wavefront: every lane reads the system and the previous pose
+// (broadcast reads) and evaluates gn_update_f64, so the factorisation waits
+// on no cross-lane exchange; lane 0 stores the new pose in FP64 (Tn) and
+// FP32 (rtf). Returns whether the factorisation succeeded (Tn and rtf are
+// left unset when not).
 __device__ inline bool wave_solve_update(const double* tot, const double* To,
                                          double* Tn, float* rtf) {
-  const int l = threadIdx.x & 63;
-  const bool elem = l < 36;
-  const bool rhs = l >= 36 && l < 42;
-  const int i = elem ? l / 6 : (rhs ? l - 36 : 0);
-  const int k = elem ? l % 6 : 0;
-  double a = elem ? sym6(tot, i, k) : (rhs ? tot[21 + i] : 0.0);
-  // Partner of lane (i, k) in the update: L[k][j] (lane 6k + j) for the
-  // matrix, y[j] (lane 36 + j) for the right-hand side.
-  const int pbase = rhs ? 36 : 6 * k;
-  double rinv[6];
-  bool ok = true;
+  double sys[27], T[12], U[12];
 #pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    const double d2 = readlane_f64(a, 7 * j);
-    ok = ok && (d2 > 0.0);
-    double r = __builtin_amdgcn_rsq(d2);
-    r = fma(r * 0.5, fma(-d2 * r, r, 1.0), r);
-    r = fma(r * 0.5, fma(-d2 * r, r, 1.0), r);
-    rinv[j] = r;
-    const double lij = __shfl(a, 6 * i + j, 64) * r;  // L[i][j]
-    const double lkj = __shfl(a, pbase + j, 64) * r;  // L[k][j] or y[j]
-    if ((elem || rhs) && i > j && (rhs || k > j)) a = a - lij * lkj;
-    if (elem && k == j && i > j) a = lij;  // this lane holds L[i][j] now
-  }
-  double y[6], x[6];
+  for (int k = 0; k < 27; ++k) sys[k] = tot[k];
 #pragma unroll
-  for (int r = 0; r < 6; ++r) y[r] = readlane_f64(a, 36 + r) * rinv[r];
+  for (int k = 0; k < 12; ++k) T[k] = To[k];
+  const bool ok = gn_update_f64(sys, T, U);
+  if (ok && (threadIdx.x & 63) == 0) {
 #pragma unroll
-  for (int r = 5; r >= 0; --r) {  // L^T x = y, the newest x[q] last
-    double v = y[r];
-#pragma unroll
-    for (int q = 5; q > r; --q) v -= readlane_f64(a, 6 * q + r) * x[q];
-    x[r] = v * rinv[r];
-  }
-  double U[12];
-  exp_rt_f64(x, U);
-  if (l < 12) {
-    const int row = l >> 2, c = l & 3;
-    const double u0 = row == 0 ? U[0] : (row == 1 ? U[4] : U[8]);
-    const double u1 = row == 0 ? U[1] : (row == 1 ? U[5] : U[9]);
-    const double u2 = row == 0 ? U[2] : (row == 1 ? U[6] : U[10]);
-    const double u3 = row == 0 ? U[3] : (row == 1 ? U[7] : U[11]);
-    double v = u0 * To[c] + u1 * To[4 + c] + u2 * To[8 + c];
-    if (c == 3) v += u3;
-    Tn[l] = v;
-    rtf[l] = float(v);
+    for (int k = 0; k < 12; ++k) {
+      Tn[k] = U[k];
+      rtf[k] = float(U[k]);
+    }
   }
   return ok;
 }
