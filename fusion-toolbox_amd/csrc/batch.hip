@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 template <typename S, bool GEOM, bool FEAT, int NB>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, 4)
     batch_jacobian_kernel(const void* __restrict__ plan,
                           stb_icp_state* __restrict__ states,
                           unsigned long long* __restrict__ maps0,

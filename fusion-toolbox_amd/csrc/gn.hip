@@ -196,11 +196,12 @@ __global__ void __launch_bounds__(BLOCK)
   for (int i = 0; i < 12; ++i) m[i] = rt[i];
   const LevelView<S> L = view_of<S>(Ld);
   const LevelRes R = res_of(L);
-  // The thread's first pixel assigns every slot (its terms, or zeros).
   float acc[kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
   if (ONE_PX) {
-    jacobian_pixel<S, GEOM, FEAT, NB, true>(L, R, m, map, tag, gtid, tw, acc,
-                                            gtid < L.n);
+    if (gtid < L.n)
+      jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, tag, gtid, tw, acc);
   } else {
     // Each block takes a contiguous run of pixels, so the pixels beyond one
     // per thread spread over every block instead of doubling the work of the
@@ -220,13 +221,12 @@ __global__ void __launch_bounds__(BLOCK)
     const V3<float> ps0 = res_ld3<S>(R.sp, int(i0));
     const uint8_t vs1 = res_u8(R.sm, int(j1));
     const V3<float> ps1 = res_ld3<S>(R.sp, int(j1));
-    jacobian_pixel_at<S, GEOM, FEAT, NB, true>(L, R, m, map, tag, i0, i0 < hi,
-                                                 vs0, ps0, tw, acc);
-    // A branch, not `live`: most wavefronts have no second pixel and skip
-    // its projection altogether.
+    if (i0 < hi)
+      jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, i0, vs0, ps0, tw,
+                                           acc);
     if (i1 < hi)
-      jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, i1, true, vs1,
-                                           ps1, tw, acc);
+      jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, i1, vs1, ps1, tw,
+                                           acc);
     for (int64_t idx = i1 + BLOCK; idx < hi; idx += BLOCK)
       jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, tw, acc);
   }

@@ -296,16 +296,7 @@ __device__ inline void corresp_pixel(const LevelView<S>& L, const LevelRes& R,
                       cstar);
 }
 
-// acc = x * y + acc, or acc = x * y when the term is the first into the slot
-// (the same value; only a zero's sign can differ from x * y + 0).
-template <bool BORN>
-__device__ inline void acc_fma(float& acc, float x, float y) {
-  acc = BORN ? x * y : fmaf(x, y, acc);
-}
-
-// Adds one weighted term J' = w J, r' (already scaled) to the JtJ/Jr slots
-// (with BORN, assigns it).
-template <bool BORN = false>
+// Adds one weighted term J' = w J, r' (already scaled) to the JtJ/Jr slots.
 __device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
                                 float r) {
   float jw[6];
@@ -316,11 +307,11 @@ __device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
   for (int a = 0; a < 6; ++a)
 #pragma unroll
     for (int b = a; b < 6; ++b) {
-      acc_fma<BORN>(acc[k], jw[a], jw[b]);
+      acc[k] = fmaf(jw[a], jw[b], acc[k]);
       ++k;
     }
 #pragma unroll
-  for (int a = 0; a < 6; ++a) acc_fma<BORN>(acc[21 + a], jw[a], r);
+  for (int a = 0; a < 6; ++a) acc[21 + a] = fmaf(jw[a], r, acc[21 + a]);
 }
 
 // The four bilinear taps of one sample from the resource of one [H, W]
@@ -516,18 +507,11 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
 // loads issue in one batch: a pixel costs two memory round trips. A pixel
 // that does not own its target adds exact zeros (selected, never multiplied
 // in, so NaN or inf in its unused data cannot leak into the sums).
-// No branch depends on one lane's validity (`live`: the pixel exists; its
-// source mask; its projection inside the image): an invalid lane evaluates
-// the terms at the projection of the wavefront's first valid lane, so its
-// loads touch lines that lane loads anyway, and adds zeros; only a
-// wavefront with no valid lane returns early. Without per-lane branches the
-// slots stay in place across the pixel, and with BORN (a thread's first
-// pixel) the terms assign the slots instead of adding to zeros.
-template <typename S, bool GEOM, bool FEAT, int NB = 4, bool BORN = false>
+template <typename S, bool GEOM, bool FEAT, int NB = 4>
 __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
                                          const LevelRes& R, const float* m,
                                          const unsigned long long* __restrict__ map,
-                                         uint32_t tag, int64_t idx, bool live,
+                                         uint32_t tag, int64_t idx,
                                          uint8_t valid_src, V3<float> ps,
                                          const TermWeights& tw,
                                          float (&acc)[kSlots]) {
@@ -535,35 +519,15 @@ __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
   const V3<float> p = rigid_point(m, ps);
   float u, v;
   project(L.fx, L.fy, L.cx, L.cy, p, u, v);
-  int ui = 0, vi = 0;
+  int ui, vi;
   const bool in_u = round_in_range(u, L.W, ui);
   const bool in_v = round_in_range(v, L.H, vi);
-  const bool ok = live && valid_src && in_u && in_v;
-  const uint64_t okm = __ballot(ok);
-  if (okm == 0) {
-    if (BORN) {
-#pragma unroll
-      for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
-    }
-    return;
-  }
-  const int first = __builtin_ctzll(okm);
-  const float uf = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u), first));
-  const float vf = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), first));
-  const int uif = __builtin_amdgcn_readlane(ui, first);
-  const int vif = __builtin_amdgcn_readlane(vi, first);
-  u = ok ? u : uf;
-  v = ok ? v : vf;
-  ui = ok ? ui : uif;
-  vi = ok ? vi : vif;
+  if (!valid_src || !in_u || !in_v) return;
   const int pos = vi * L.W + ui;
   const __amdgpu_buffer_rsrc_t rmap = make_res(map, L.HW);
   const uint32_t klo = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos, 0, 0);
   const uint32_t khi = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos + 4, 0, 0);
-  // `&`, not `&&`: a short-circuit would put the map load under a branch that
-  // waits for it before the target loads issue.
-  const bool own =
-      ok & key_owned((uint64_t(khi) << 32) | klo, uint32_t(idx), tag);
+  const bool own = key_owned((uint64_t(khi) << 32) | klo, uint32_t(idx), tag);
   if (GEOM) {
     const V3<float> q = res_ld3<S>(R.tp, pos);
     const V3<float> qn = res_ld3<S>(R.tn, pos);
@@ -572,42 +536,35 @@ __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
     const float j[6] = {own ? qn.x : 0.0f, own ? qn.y : 0.0f, own ? qn.z : 0.0f,
                         own ? c.x : 0.0f,  own ? c.y : 0.0f,  own ? c.z : 0.0f};
     const float rs = own ? r : 0.0f;
-    acc_term<BORN>(acc, j, tw.wg, rs * tw.sg);
-    acc_fma<BORN>(acc[27], rs, rs);
-    acc[29] = (BORN ? 0.0f : acc[29]) + (own ? 1.0f : 0.0f);
+    acc_term(acc, j, tw.wg, rs * tw.sg);
+    acc[27] = fmaf(rs, rs, acc[27]);
+    acc[29] += own ? 1.0f : 0.0f;
   }
   if (FEAT) {
-    constexpr bool kBornF = BORN && !GEOM;
     float jf[6], rl;
     const float sq = feature_terms_res<S, NB>(L, idx, p, u, v, jf, rl);
     const bool use = own && !(sq > tw.fthr2);
     float j[6];
 #pragma unroll
     for (int a = 0; a < 6; ++a) j[a] = use ? jf[a] : 0.0f;
-    acc_term<kBornF>(acc, j, tw.wf, (use ? rl : 0.0f) * tw.sf);
-    acc[28] = (kBornF ? 0.0f : acc[28]) + (use ? sq : 0.0f);
-    acc[30] = (kBornF ? 0.0f : acc[30]) + (use ? 1.0f : 0.0f);
-  }
-  if (BORN) {
-    if (!GEOM) acc[27] = acc[29] = 0.0f;
-    if (!FEAT) acc[28] = acc[30] = 0.0f;
-    acc[31] = 0.0f;
+    acc_term(acc, j, tw.wf, (use ? rl : 0.0f) * tw.sf);
+    acc[28] += use ? sq : 0.0f;
+    acc[30] += use ? 1.0f : 0.0f;
   }
 }
 
-// jacobian_pixel_at with the source pixel's mask and point loaded here
-// (past the arrays' ends the buffer loads return 0).
-template <typename S, bool GEOM, bool FEAT, int NB = 4, bool BORN = false>
+// jacobian_pixel_at with the source pixel's mask and point loaded here.
+template <typename S, bool GEOM, bool FEAT, int NB = 4>
 __device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
                                       const float* m,
                                       const unsigned long long* __restrict__ map,
                                       uint32_t tag, int64_t idx,
                                       const TermWeights& tw,
-                                      float (&acc)[kSlots], bool live = true) {
+                                      float (&acc)[kSlots]) {
   const uint8_t valid_src = res_u8(R.sm, int(idx));
   const V3<float> ps = res_ld3<S>(R.sp, int(idx));
-  jacobian_pixel_at<S, GEOM, FEAT, NB, BORN>(L, R, m, map, tag, idx, live,
-                                             valid_src, ps, tw, acc);
+  jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, valid_src, ps,
+                                       tw, acc);
 }
 
 // G^T G of source pixel idx from its matched target point

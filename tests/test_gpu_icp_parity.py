@@ -249,6 +249,51 @@ def test_icp_features_matches_oracle():
     assert err[0] < 1e-6 and err[1] < 1e-6
 
 
+def test_literal_one_to_point_one_weights_match_the_oracle_step_by_step():
+    """The 1 : 0.1 geometric : feature weighting written literally as
+    (geom_weight, feat_weight) = (1, 0.1) rather than the (10, 1) the bench
+    uses: with JtJ scaled by w^2 and Jr by w (slamtb/registration/icp.py:
+    221-234) the literal form's iteration is chaotic, so free-running poses
+    part from any other implementation's after a few steps. Parity is shown
+    decision-matched: each of 4 iterations at 320x240 with 3 feature
+    channels takes the oracle's step from the GPU's pose before it, with the
+    oracle's match count and its pose to 1e-8."""
+    P = prepared_pair(5, 320, 240, channels=3)
+    g = to_dev(P, DEV)
+    level = Level(P["kcam"], g["src_pts"], g["src_nrm"], g["src_mask"],
+                  g["tgt_pts"], g["tgt_nrm"], g["tgt_mask"], g["src_feats"],
+                  g["tgt_feats"])
+    gw, fw = 1.0, 0.1
+    params = make_params(gw, fw, 0.1, ANG20, 2.75, True)
+    args = (P["kcam"], P["src_pts"], P["src_nrm"], P["src_mask"], P["tgt_pts"],
+            P["tgt_nrm"], P["tgt_mask"])
+    kw = dict(src_feats=P["src_feats"], tgt_feats=P["tgt_feats"],
+              geom_weight=gw, feat_weight=fw, distance_threshold=0.1,
+              normals_angle_thresh=ANG20, feat_residual_thresh=2.75,
+              with_information=False)
+    run = runner_for(DEV)
+    run.begin(level, torch.eye(4, dtype=torch.float64))
+    prev = np.eye(4)
+    worst = 0.0
+    for i in range(4):
+        run.correspond(level, params)
+        run.update(level, params)
+        st = run.read_state()
+        ref = orc.icp_estimate(*args, 1, transform=prev, **kw)
+        assert st.iterations == i + 1 and st.stopped == 0
+        assert max(st.last_geom_count, st.last_feat_count) == ref["match_count"], i
+        cur = state_pose(st)
+        err = orc.pose_error(cur, ref["transform"])
+        step = orc.pose_error(prev, cur)
+        print(f"iteration {i + 1}: step {step}, difference from the oracle's step {err}")
+        worst = max(worst, *err)
+        assert err[0] < 1e-8 and err[1] < 1e-8, (i, err)
+        prev = cur
+    free = orc.icp_estimate(*args, 4, **kw)
+    print("worst decision-matched difference", worst, "; free-running 4 iterations",
+          orc.pose_error(prev, free["transform"]))
+
+
 def test_icp_zero_overlap_keeps_no_transform(pair320):
     far = np.eye(4)
     far[0, 3] = 100.0  # every source point leaves the image
