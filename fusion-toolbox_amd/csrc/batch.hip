@@ -87,10 +87,23 @@ __global__ void __launch_bounds__(kBlock, 4)
     float acc[kSlots];
 #pragma unroll
     for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
+    // The next pixel's source mask and point load while this pixel's gathers
+    // and terms run, so a pixel waits for one memory round trip instead of
+    // two. Past the tile (another tile's pixel) or the arrays' ends (the
+    // buffer loads return 0) the prefetch is simply unused.
+    int64_t idx = base + threadIdx.x;
+    uint8_t vs = res_u8(R.sm, int(idx));
+    V3<float> ps = res_ld3<S>(R.sp, int(idx));
     for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
-      const int64_t idx = base + k;
+      const int64_t nxt = idx + blockDim.x;
+      const uint8_t vs_n = res_u8(R.sm, int(nxt));
+      const V3<float> ps_n = res_ld3<S>(R.sp, int(nxt));
       if (idx < L.n)
-        jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, tw, acc);
+        jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, vs, ps,
+                                             tw, acc);
+      idx = nxt;
+      vs = vs_n;
+      ps = ps_n;
     }
     block_partial(acc, partials + int64_t(t) * kSlots, sh);
   }
@@ -224,11 +237,14 @@ void launch_batch(const void* plan, const stb_icp_batch_sizes& z,
                   hipStream_t s) {
   const float dthr2 = float(P.distance_thresh * P.distance_thresh);
   const float cstar = angle_gate_cos_f32(P.normal_angle_thresh);
+  // By default one block per tile in both launches (64 mixed pairs, tiles of
+  // 1024 pixels: 8,220 pairs/s, against 7,880 with 4,096 blocks looping over
+  // the tiles and 7,680 with 1,024).
   const int gc = (P.grid_corresp > 0) ? (z.total_tiles < P.grid_corresp
                                              ? z.total_tiles
                                              : P.grid_corresp)
-                                      : (z.total_tiles < 8192 ? z.total_tiles : 8192);
-  const int gj = grid_tiles(&P, z.total_tiles, 1024);
+                                      : z.total_tiles;
+  const int gj = grid_tiles(&P, z.total_tiles, z.total_tiles);
   const TermWeights tw = make_weights(P);
   const int fixed = phase == kIterate ? 0 : 1;
   const int reps = phase == kIterate ? count : 1;
@@ -300,7 +316,7 @@ void launch_batch_correspond_f32(const void* plan,
                                  const Scratch& sc, hipStream_t s) {
   const float dthr2 = float(P.distance_thresh * P.distance_thresh);
   const float cstar = angle_gate_cos_f32(P.normal_angle_thresh);
-  const int gc = z.total_tiles < 8192 ? z.total_tiles : 8192;
+  const int gc = z.total_tiles;  // one block per tile
   hipLaunchKernelGGL(batch_corresp_kernel<float>, dim3(gc), dim3(kBlock), 0, s,
                      plan, states, sc.map0, sc.map1, dthr2, cstar, 0);
 }

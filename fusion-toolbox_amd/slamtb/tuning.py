@@ -78,7 +78,7 @@ def tune_batch(estimators, pairs, device, reps=5):
     from slamtb.registration.batch import BatchedMultiscaleICP
     best, best_t = None, float("inf")
     for tile in TILE_CANDIDATES:
-        for grid in (1024, 2048, 4096):
+        for grid in (1024, 4096, 16384, 1 << 20):  # 1 << 20: one block per tile
             b = BatchedMultiscaleICP(estimators, pairs, device, tile_px=tile,
                                      grid_update=grid, grid_corresp=grid)
             t = _time(b.enqueue, reps)
