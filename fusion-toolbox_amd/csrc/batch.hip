@@ -3,10 +3,11 @@
 // A batch plan (built on the host by stb_icp_batch_plan, uploaded by the
 // caller) lists one level descriptor per pair and cuts every pair's source
 // pixels into equal tiles. Each iteration is three launches whatever the pair
-// count or resolutions: correspondence and Jacobians are persistent launches
-// over the flattened tile list of all pairs (a workgroup walks tiles
-// t = blockIdx.x, blockIdx.x + gridDim.x, ...; every Jacobian tile leaves one
-// FP64 partial), then one workgroup per pair sums that pair's partials in
+// count or resolutions: correspondence and Jacobians are launches over the
+// flattened tile list of all pairs (by default one workgroup per tile; with a
+// grid cap a workgroup walks tiles t = blockIdx.x, blockIdx.x + gridDim.x,
+// ...; every Jacobian tile leaves one FP64 partial), then one workgroup per
+// pair sums that pair's partials in
 // tile order, solves its 6x6 system in FP64 and updates its pose. All pairs
 // advance one iteration per three launches with no host involvement, and a
 // whole multi-level loop can be captured in one hipGraph.
