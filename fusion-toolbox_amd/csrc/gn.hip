@@ -45,7 +45,8 @@ __global__ void __launch_bounds__(kBlock)
   for (int i = 0; i < 9; ++i) nm[i] = st->nm[i];
   const LevelView<S> L = view_of<S>(Ld);
   const LevelRes R = res_of(L);
-  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < L.n;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  for (int64_t idx = int64_t(blk) * blockDim.x + threadIdx.x; idx < L.n;
        idx += int64_t(gridDim.x) * blockDim.x)
     corresp_pixel(L, R, m, nm, idx, map, tag, dthr2, cstar);
 }
@@ -79,7 +80,7 @@ __global__ void __launch_bounds__(kSolveBlock)
   const LevelView<S> L = view_of<S>(Ld);
   // A contiguous run of pixels per block (see jacobian_fused_kernel).
   const int64_t chunk = (L.n + gridDim.x - 1) / gridDim.x;
-  const int64_t lo = int64_t(blockIdx.x) * chunk;
+  const int64_t lo = int64_t(xcd_block(blockIdx.x, gridDim.x)) * chunk;
   const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
   const int64_t idx = lo + t;
   // The partials first: the sum waits only for them, the loads below stay
@@ -190,7 +191,10 @@ __global__ void __launch_bounds__(BLOCK)
   const uint32_t tag =
       fixed_map ? kFreshTag : uint32_t(rec ? rec->map_epoch : st->map_epoch);
   const unsigned long long* __restrict__ map = (sel ^ fixed_map) ? map1 : map0;
-  const int64_t gtid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  // Block `blk` takes the blk-th run of pixels and leaves the blk-th partial
+  // (XCD-contiguous runs; the partials keep the image order).
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t gtid = int64_t(blk) * blockDim.x + threadIdx.x;
   float m[12];
 #pragma unroll
   for (int i = 0; i < 12; ++i) m[i] = rt[i];
@@ -213,7 +217,7 @@ __global__ void __launch_bounds__(BLOCK)
     // pixel re-reads its first: the next run belongs to a block on another
     // XCD, whose lines would cross the fabric a second time.
     const int64_t chunk = (L.n + gridDim.x - 1) / gridDim.x;
-    const int64_t lo = int64_t(blockIdx.x) * chunk;
+    const int64_t lo = int64_t(blk) * chunk;
     const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
     const int64_t i0 = lo + threadIdx.x, i1 = i0 + BLOCK;
     const int64_t j1 = i1 < hi ? i1 : i0;
@@ -235,11 +239,11 @@ __global__ void __launch_bounds__(BLOCK)
     // iteration uses with a fresh tag (one-off passes may have left it set).
     unsigned long long* __restrict__ next = sel ? map0 : map1;
     const int64_t chunk = (map_len + gridDim.x - 1) / gridDim.x;
-    const int64_t lo = int64_t(blockIdx.x) * chunk;
+    const int64_t lo = int64_t(blk) * chunk;
     const int64_t hi = lo + chunk < map_len ? lo + chunk : map_len;
     for (int64_t i = lo + threadIdx.x; i < hi; i += BLOCK) next[i] = kKeySentinel;
   }
-  block_partial<BLOCK>(acc, partials + int64_t(blockIdx.x) * kSlots, sh);
+  block_partial<BLOCK>(acc, partials + int64_t(blk) * kSlots, sh);
   // Last, so that its round trip does not delay block 0's pixels.
   if (rec && blockIdx.x == 0) commit_record(st, rec, tw.gw, tw.fw);
 }
@@ -275,10 +279,11 @@ __global__ void __launch_bounds__(kBlock)
   float acc[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
-  for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < L.n;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  for (int64_t idx = int64_t(blk) * blockDim.x + threadIdx.x; idx < L.n;
        idx += int64_t(gridDim.x) * blockDim.x)
     information_pixel(L, m, map, kFreshTag, idx, acc);
-  block_partial(acc, partials + int64_t(blockIdx.x) * kSlots, sh);
+  block_partial(acc, partials + int64_t(blk) * kSlots, sh);
 }
 
 __global__ void icp_begin_kernel(stb_icp_state* st,

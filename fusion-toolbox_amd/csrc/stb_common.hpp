@@ -236,6 +236,30 @@ __host__ __device__ inline MapUse next_map_use(uint32_t tag, int sel) {
 
 constexpr int kBlock = 256;
 
+// MI355X hands the workgroups of a launch to its 8 XCDs in turn (workgroup i
+// to XCD i % 8), and each XCD has an L2 of its own. xcd_block(i, g) numbers
+// the g workgroups so that each XCD's share is one contiguous run of work
+// items: neighbouring pixels, whose gathers share target cache lines, then
+// meet in one L2 instead of being fetched by two. A permutation of [0, g):
+// XCD x takes the q or q + 1 consecutive numbers after those of XCDs < x.
+constexpr int kXcds = 8;
+__device__ inline int xcd_block(int i, int g) {
+  const int x = i % kXcds, j = i / kXcds;
+  const int q = g / kXcds, r = g % kXcds;
+  return x * q + (x < r ? x : r) + j;
+}
+
+// The same in chunks of c consecutive work items that rotate over the XCDs
+// (chunk k on XCD k % 8): neighbours still share an L2, and work items of
+// unequal cost spread over every XCD. Workgroups past the last whole round
+// of 8 chunks keep their own numbers, so the map is a permutation of [0, g).
+__device__ inline int xcd_chunk(int i, int g, int c) {
+  const int full = g / (kXcds * c) * (kXcds * c);
+  if (i >= full) return i;
+  const int x = i % kXcds, j = i / kXcds;
+  return ((j / c) * kXcds + x) * c + j % c;
+}
+
 inline int grid_for(int64_t n, int block = kBlock, int max_blocks = 1 << 20) {
   int64_t g = (n + block - 1) / block;
   if (g < 1) g = 1;
