@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(kSolveBlock)
   __shared__ double sh[kSolveBlock];
   __shared__ double tot[kSlots];
   __shared__ double To[12], Tn[12];
-  __shared__ float rtf[12];
+  __shared__ float rtf[12], nmf[9];
   __shared__ int s_ok;
   const int t = threadIdx.x;
   const LevelView<S> L = view_of<S>(Ld);
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(kSolveBlock)
   if (t < 12) To[t] = row;
   __syncthreads();
   if (t < 64) {
-    const bool good = wave_solve_update(tot, To, Tn, rtf);
+    const bool good = wave_solve_update(tot, To, Tn, rtf, nmf);
     if (t == 0) s_ok = good;
   }
   const int stopped = __shfl(flag, 0, 64);
@@ -130,7 +130,8 @@ __global__ void __launch_bounds__(kSolveBlock)
   if (ok) {
 #pragma unroll
     for (int i = 0; i < 12; ++i) r.m[i] = rtf[i];
-    make_normal_matrix(r);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r.nm[i] = nmf[i];
   } else {  // stopped before, or stopping now: the pose stays
 #pragma unroll
     for (int i = 0; i < 12; ++i) r.m[i] = st->rt[i];

@@ -1081,11 +1081,12 @@ struct SolveRecord {
 // The update of a loop iteration from the summed system in LDS, by one
 // whole wavefront: every lane reads the system and the previous pose
 // (broadcast reads) and evaluates gn_update_f64, so the factorisation waits
-// on no cross-lane exchange; lane 0 stores the new pose in FP64 (Tn) and
-// FP32 (rtf). Returns whether the factorisation succeeded (Tn and rtf are
-// left unset when not).
+// on no cross-lane exchange; lane 0 stores the new pose in FP64 (Tn), in
+// FP32 (rtf) and that pose's normal matrix (nmf), once for the whole block.
+// Returns whether the factorisation succeeded (Tn, rtf and nmf are left
+// unset when not).
 __device__ inline bool wave_solve_update(const double* tot, const double* To,
-                                         double* Tn, float* rtf) {
+                                         double* Tn, float* rtf, float* nmf) {
   double sys[27], T[12], U[12];
 #pragma unroll
   for (int k = 0; k < 27; ++k) sys[k] = tot[k];
@@ -1093,11 +1094,16 @@ __device__ inline bool wave_solve_update(const double* tot, const double* To,
   for (int k = 0; k < 12; ++k) T[k] = To[k];
   const bool ok = gn_update_f64(sys, T, U);
   if (ok && (threadIdx.x & 63) == 0) {
+    Rigid<float> r;
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
       Tn[k] = U[k];
-      rtf[k] = float(U[k]);
+      r.m[k] = float(U[k]);
+      rtf[k] = r.m[k];
     }
+    make_normal_matrix(r);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) nmf[k] = r.nm[k];
   }
   return ok;
 }
