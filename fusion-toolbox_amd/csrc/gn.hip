@@ -215,8 +215,7 @@ __global__ void __launch_bounds__(BLOCK)
     // The first two pixels' source loads issue together, so a thread's
     // second pixel does not wait a round trip behind its first (past the end
     // of the arrays the buffer loads return 0). A thread without a second
-    // pixel re-reads its first: the next run belongs to a block on another
-    // XCD, whose lines would cross the fabric a second time.
+    // pixel re-reads its first instead of touching the next block's run.
     const int64_t chunk = (L.n + gridDim.x - 1) / gridDim.x;
     const int64_t lo = int64_t(blk) * chunk;
     const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
