@@ -21,8 +21,9 @@ synchronisations. Also reported:
   * per-launch durations of the correspondence and Jacobian kernels (HIP
     events around a captured graph of back-to-back launches of that kernel
     alone) and the dominant kernel's roofline against 8 TB/s, with the HBM
-    bytes that rocprofv3 counters measured for it (profiles/) and a
-    device-to-device copy ceiling measured in the same process;
+    bytes that rocprofv3 counters measured for it (profiles/), a
+    device-to-device copy ceiling measured in the same process, and the
+    whole iteration's fraction of 8 TB/s beside it;
   * the full 3-level registration per pair, its pose error against the
     oracle fed the same level inputs -- FP32 per-pixel arithmetic with FP64
     sums (decision-matched) and all-FP64 -- and against the ground truth;
@@ -1123,7 +1124,17 @@ def main():
                      "bytes_per_launch": dom["bytes"],
                      "bytes_per_launch_all_pixels": dom["bytes_all"],
                      "achieved_all_pixels": dom["bytes_all"] / (dom["ms"] * 1e-3) / 1e9,
-                     "copy_ceiling_gbs": copy_gbs},
+                     "copy_ceiling_gbs": copy_gbs,
+                     # The whole loop iteration (both launches and the gap
+                     # between them) over the same algorithmic bytes.
+                     "iteration_bytes": bytes_a + bytes_b,
+                     "iteration_bytes_all_pixels": bytes_a_all + bytes_b_all,
+                     "iteration_achieved": (bytes_a + bytes_b) / (ms_per_step * 1e-3) / 1e9,
+                     "iteration_frac": (bytes_a + bytes_b) / (ms_per_step * 1e-3) / 1e9
+                                       / HBM_PEAK_GBS,
+                     "iteration_frac_all_pixels": (bytes_a_all + bytes_b_all)
+                                                  / (ms_per_step * 1e-3) / 1e9
+                                                  / HBM_PEAK_GBS},
         # A loop iteration is a correspondence launch that first applies the
         # previous update (corresp_solve_kernel) and the Jacobian launch; the
         # plain correspondence kernel runs only in a loop's first iteration.
