@@ -653,7 +653,10 @@ __device__ inline void halve_dpp(float (&v)[kSlots], bool up) {
   }
 }
 
-// After this, lane l holds the wavefront total of slot l >> 1 in v[0].
+// After this, lane l holds the wavefront total of slot l >> 1 in v[0]. The
+// swaps and DPP reads take partner lanes' registers whatever the exec mask
+// says, so every lane of the wavefront must be active (block_partial is
+// reached by whole blocks of full wavefronts).
 __device__ inline void wave_reduce_slots(float (&v)[kSlots], int lane) {
   halve_swap32(v);
   halve_swap16(v);
