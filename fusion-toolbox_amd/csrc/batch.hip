@@ -70,16 +70,23 @@ __global__ void __launch_bounds__(kBlock, 4)
   for (int t = xcd_chunk(blockIdx.x, gridDim.x, kTileChunk); t < pv.tiles;
        t += gridDim.x) {
     const int p = pv.tile_pair[t];
-    stb_icp_state* st = states + p;
-    if (!fixed_map && st->stopped) continue;
     const LevelView<S> L = view_of<S>(pv.levels[p]);
     const LevelRes R = res_of(L);
-    const int64_t off = pv.map_off[p];
-    const unsigned long long* map = pair_map(st, maps0, maps1, off, fixed_map);
-    const uint32_t tag = fixed_map ? kFreshTag : uint32_t(st->map_epoch);
     const int first = pv.tile_start[p];
     const int ntiles = pv.tile_start[p + 1] - first;
     const int64_t base = int64_t(t - first) * pv.tile_px;
+    // The tile's first source pixel loads before the pair's state is read
+    // (the state was written by the previous launch and misses this XCD's
+    // L2); for a stopped pair the loads are simply unused.
+    int64_t idx = base + threadIdx.x;
+    uint8_t vs = res_u8(R.sm, int(idx));
+    V3<float> ps = res_ld3<S>(R.sp, int(idx));
+    __builtin_amdgcn_sched_barrier(0);
+    stb_icp_state* st = states + p;
+    if (!fixed_map && st->stopped) continue;
+    const int64_t off = pv.map_off[p];
+    const unsigned long long* map = pair_map(st, maps0, maps1, off, fixed_map);
+    const uint32_t tag = fixed_map ? kFreshTag : uint32_t(st->map_epoch);
     if (!fixed_map && tag == 0) {
       // The pair's last tag on this map: clear this tile's share of the
       // other map, which the next iteration uses with a fresh tag; the
@@ -95,27 +102,35 @@ __global__ void __launch_bounds__(kBlock, 4)
 #pragma unroll
     for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
     float acc[kSlots];
-#pragma unroll
-    for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
     // The next pixel's source mask and point load while this pixel's gathers
     // and terms run, so a pixel waits for one memory round trip instead of
     // two. Past the tile (another tile's pixel) or the arrays' ends (the
-    // buffer loads return 0) the prefetch is simply unused.
-    int64_t idx = base + threadIdx.x;
-    uint8_t vs = res_u8(R.sm, int(idx));
-    V3<float> ps = res_ld3<S>(R.sp, int(idx));
-    for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
+    // buffer loads return 0) the prefetch is simply unused. The first pixel
+    // starts the slots (every lane takes part; lanes past the tile or the
+    // pair's pixels add zeros).
+    {
       const int64_t nxt = idx + blockDim.x;
       const uint8_t vs_n = res_u8(R.sm, int(nxt));
       const V3<float> ps_n = res_ld3<S>(R.sp, int(nxt));
-      if (idx < L.n)
-        jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, vs, ps,
-                                             tw, acc);
+      const bool ok = (int(threadIdx.x) < pv.tile_px) & (idx < L.n) & (vs != 0);
+      jacobian_pixel_at<S, GEOM, FEAT, true, NB>(L, R, m, map, tag, idx, ok,
+                                                 ps, tw, acc);
       idx = nxt;
       vs = vs_n;
       ps = ps_n;
     }
-    block_partial(acc, partials + int64_t(t) * kSlots, sh);
+    for (int k = threadIdx.x + blockDim.x; k < pv.tile_px; k += blockDim.x) {
+      const int64_t nxt = idx + blockDim.x;
+      const uint8_t vs_n = res_u8(R.sm, int(nxt));
+      const V3<float> ps_n = res_ld3<S>(R.sp, int(nxt));
+      if (idx < L.n)
+        jacobian_pixel_at<S, GEOM, FEAT, false, NB>(L, R, m, map, tag, idx,
+                                                    vs != 0, ps, tw, acc);
+      idx = nxt;
+      vs = vs_n;
+      ps = ps_n;
+    }
+    block_partial(acc, partials + int64_t(t) * kSlots, sh, tw.b);
   }
 }
 

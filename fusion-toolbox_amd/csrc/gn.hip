@@ -170,12 +170,14 @@ __global__ void __launch_bounds__(kSolveBlock)
 
 // Per-pixel Jacobians of one iteration; every block leaves one FP64 partial
 // for the launch that sums them, so no block waits on another. With ONE_PX
-// each thread owns exactly one pixel (levels that fit the grid cap): the
-// slot accumulators are then born from that pixel's terms instead of living
-// across a loop (~84 VGPRs instead of ~116). Larger levels use the
-// grid-stride loop. With `rec` (iterations after the first of a loop) the
-// pose and map come from the record of this iteration's correspondence
-// launch, and block 0 commits that record to the state.
+// each thread owns at most one pixel (levels that fit the grid cap); larger
+// levels give block `blk` the blk-th run of `chunk` pixels (chunk computed
+// on the host). With `rec` (iterations after the first of a loop) the pose
+// and map come from the record of this iteration's correspondence launch,
+// and block 0 commits that record to the state. The source loads issue
+// before any read of the state or the record: those were written by the
+// previous launch, miss this XCD's L2, and would otherwise hold the first
+// memory round trip back by one of their own.
 template <typename S, bool GEOM, bool FEAT, bool ONE_PX, int NB, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
     jacobian_fused_kernel(stb_icp_level Ld, stb_icp_state* __restrict__ st,
@@ -183,67 +185,68 @@ __global__ void __launch_bounds__(BLOCK)
                           unsigned long long* __restrict__ map0,
                           unsigned long long* __restrict__ map1,
                           int64_t map_len, double* __restrict__ partials,
-                          TermWeights tw, int fixed_map) {
+                          TermWeights tw, int fixed_map, int64_t chunk) {
   __shared__ double sh[(BLOCK / 64) * kSlots];
-  // No early exit on state.stopped (see corresp_fused_kernel): the solve
-  // ignores the partials of a stopped loop.
-  const int sel = rec ? rec->map_sel : st->map_sel;
-  const float* __restrict__ rt = rec ? rec->rt : st->rt;
-  const uint32_t tag =
-      fixed_map ? kFreshTag : uint32_t(rec ? rec->map_epoch : st->map_epoch);
-  const unsigned long long* __restrict__ map = (sel ^ fixed_map) ? map1 : map0;
   // Block `blk` takes the blk-th run of pixels and leaves the blk-th partial
   // (XCD-contiguous runs; the partials keep the image order).
   const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t gtid = int64_t(blk) * blockDim.x + threadIdx.x;
+  const LevelView<S> L = view_of<S>(Ld);
+  const LevelRes R = res_of(L);
+  // Each block takes a contiguous run of pixels, so the pixels beyond one
+  // per thread spread over every block instead of doubling the work of the
+  // first ones (with one 1024-thread block per CU, a grid-stride loop left
+  // 44 CUs with twice the work at 640x480). A thread's first two pixels load
+  // their source data together, so its second pixel does not wait a round
+  // trip behind its first (past the end of the arrays the buffer loads
+  // return 0); a thread without a second pixel re-reads its first instead of
+  // touching the next block's run.
+  const int64_t lo = ONE_PX ? int64_t(blk) * BLOCK : int64_t(blk) * chunk;
+  const int64_t hi = ONE_PX ? L.n : (lo + chunk < L.n ? lo + chunk : L.n);
+  const int64_t i0 = lo + threadIdx.x, i1 = i0 + BLOCK;
+  const int64_t j1 = i1 < hi ? i1 : i0;
+  const uint8_t vs0 = res_u8(R.sm, int(i0));
+  const V3<float> ps0 = res_ld3<S>(R.sp, int(i0));
+  uint8_t vs1 = 0;
+  V3<float> ps1 = {0.0f, 0.0f, 0.0f};
+  if (!ONE_PX) {
+    vs1 = res_u8(R.sm, int(j1));
+    ps1 = res_ld3<S>(R.sp, int(j1));
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  // No early exit on state.stopped (see corresp_fused_kernel): the solve
+  // ignores the partials of a stopped loop. The record's or the state's
+  // words through selected pointers: independent loads in one round trip.
+  const int32_t* __restrict__ selp = rec ? &rec->map_sel : &st->map_sel;
+  const int32_t* __restrict__ tagp = rec ? &rec->map_epoch : &st->map_epoch;
+  const float* __restrict__ rt = rec ? rec->rt : st->rt;
+  const int sel = *selp;
+  const uint32_t tag = fixed_map ? kFreshTag : uint32_t(*tagp);
+  const unsigned long long* __restrict__ map = (sel ^ fixed_map) ? map1 : map0;
   float m[12];
 #pragma unroll
   for (int i = 0; i < 12; ++i) m[i] = rt[i];
-  const LevelView<S> L = view_of<S>(Ld);
-  const LevelRes R = res_of(L);
   float acc[kSlots];
-#pragma unroll
-  for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
-  if (ONE_PX) {
-    if (gtid < L.n)
-      jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, tag, gtid, tw, acc);
-  } else {
-    // Each block takes a contiguous run of pixels, so the pixels beyond one
-    // per thread spread over every block instead of doubling the work of the
-    // first ones (with one 1024-thread block per CU, a grid-stride loop left
-    // 44 CUs with twice the work at 640x480).
-    // The first two pixels' source loads issue together, so a thread's
-    // second pixel does not wait a round trip behind its first (past the end
-    // of the arrays the buffer loads return 0). A thread without a second
-    // pixel re-reads its first instead of touching the next block's run.
-    const int64_t chunk = (L.n + gridDim.x - 1) / gridDim.x;
-    const int64_t lo = int64_t(blk) * chunk;
-    const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
-    const int64_t i0 = lo + threadIdx.x, i1 = i0 + BLOCK;
-    const int64_t j1 = i1 < hi ? i1 : i0;
-    const uint8_t vs0 = res_u8(R.sm, int(i0));
-    const V3<float> ps0 = res_ld3<S>(R.sp, int(i0));
-    const uint8_t vs1 = res_u8(R.sm, int(j1));
-    const V3<float> ps1 = res_ld3<S>(R.sp, int(j1));
-    if (i0 < hi)
-      jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, i0, vs0, ps0, tw,
-                                           acc);
+  jacobian_pixel_at<S, GEOM, FEAT, true, NB>(L, R, m, map, tag, i0,
+                                             (i0 < hi) & (vs0 != 0), ps0, tw,
+                                             acc);
+  if (!ONE_PX) {
     if (i1 < hi)
-      jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, i1, vs1, ps1, tw,
-                                           acc);
+      jacobian_pixel_at<S, GEOM, FEAT, false, NB>(L, R, m, map, tag, i1,
+                                                  vs1 != 0, ps1, tw, acc);
     for (int64_t idx = i1 + BLOCK; idx < hi; idx += BLOCK)
-      jacobian_pixel<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, tw, acc);
+      jacobian_pixel<S, GEOM, FEAT, false, NB>(L, R, m, map, tag, idx, true,
+                                               tw, acc);
   }
   if (!fixed_map && tag == 0) {
     // The last tag of this map: sweep the other one, which the next
     // iteration uses with a fresh tag (one-off passes may have left it set).
     unsigned long long* __restrict__ next = sel ? map0 : map1;
-    const int64_t chunk = (map_len + gridDim.x - 1) / gridDim.x;
-    const int64_t lo = int64_t(blk) * chunk;
-    const int64_t hi = lo + chunk < map_len ? lo + chunk : map_len;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += BLOCK) next[i] = kKeySentinel;
+    const int64_t mchunk = (map_len + gridDim.x - 1) / gridDim.x;
+    const int64_t mlo = int64_t(blk) * mchunk;
+    const int64_t mhi = mlo + mchunk < map_len ? mlo + mchunk : map_len;
+    for (int64_t i = mlo + threadIdx.x; i < mhi; i += BLOCK) next[i] = kKeySentinel;
   }
-  block_partial<BLOCK>(acc, partials + int64_t(blk) * kSlots, sh);
+  block_partial<BLOCK>(acc, partials + int64_t(blk) * kSlots, sh, tw.b);
   // Last, so that its round trip does not delay block 0's pixels.
   if (rec && blockIdx.x == 0) commit_record(st, rec, tw.gw, tw.fw);
 }
@@ -429,21 +432,22 @@ void launch_jacobian_nb(const stb_icp_level& L, const JacShape& js,
                         const Workspace& w, const SolveRecord* rec,
                         const TermWeights& tw, int fixed_map,
                         hipStream_t stream) {
+  const int64_t chunk = (L.n_src + js.grid - 1) / js.grid;
   if (js.block == kSolveBlock)
     hipLaunchKernelGGL(
         (jacobian_fused_kernel<S, GEOM, FEAT, false, NB, kSolveBlock>),
         dim3(js.grid), dim3(kSolveBlock), 0, stream, L, w.st, rec, w.map[0],
-        w.map[1], w.map_len, w.partials, tw, fixed_map);
+        w.map[1], w.map_len, w.partials, tw, fixed_map, chunk);
   else if (js.one_px)
     hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, true, NB, kBlock>),
                        dim3(js.grid), dim3(kBlock), 0, stream, L, w.st, rec,
                        w.map[0], w.map[1], w.map_len, w.partials, tw,
-                       fixed_map);
+                       fixed_map, chunk);
   else
     hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, false, NB, kBlock>),
                        dim3(js.grid), dim3(kBlock), 0, stream, L, w.st, rec,
                        w.map[0], w.map[1], w.map_len, w.partials, tw,
-                       fixed_map);
+                       fixed_map, chunk);
 }
 
 // The Jacobian launch, then (do_solve >= 0) the launch that sums its

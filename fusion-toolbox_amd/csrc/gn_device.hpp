@@ -26,23 +26,33 @@ constexpr int kShared = (kGroups > kWaves ? kGroups : kWaves) * kSlots;
 // 32 slot groups share the partial loads.
 constexpr int kSolveBlock = 1024;
 
-// J scales and residual signs of the combined system of
-// slamtb/registration/icp.py:221-234: JtJ = fw^2 JtJ_f + gw^2 JtJ_g and
-// Jr = -fw Jr_f + gw Jr_g, i.e. J' = w J with r' = r (geometric) or -r
-// (feature).
+// Weights of the combined system of slamtb/registration/icp.py:221-234:
+// JtJ = gw^2 JtJ_g + fw^2 JtJ_f and Jr = gw Jr_g - fw Jr_f. The per-pixel
+// sums leave the weights out: with b = gw (or fw when the geometric term is
+// off) and rho = fw / b, a lane accumulates J_g J_g^T + (rho J_f)(rho J_f)^T
+// and J_g r_g - (rho J_f) r_f, the feature J picking up rho through the
+// divisor of its finite differences (fdiv = rho / (2h)), and the block
+// partials scale the JtJ slots by b^2 and the Jr slots by b in FP64. That
+// saves the per-pixel J scaling (12 multiplies).
 struct TermWeights {
-  float wg, sg;   // geometric J scale and residual scale
-  float wf, sf;   // feature J scale and residual scale
+  float fdiv;     // feature finite-difference divisor times rho
+  int gon;        // geometric J enters the system (gw != 0)
   float fthr2;    // feature residual threshold squared
+  double b;       // weight the partials restore: JtJ slots x b^2, Jr x b
   double gw, fw;  // weights of the residual bookkeeping
 };
 
+// The feature finite-difference step h (src/feature_map.hpp:82); the
+// derivative divides by 2h as src/feature_map.hpp:57 forms it in FP32.
+constexpr float kFeatStep = 0.05f;
+
 inline TermWeights make_weights(const stb_icp_params& P) {
   TermWeights tw;
-  tw.wg = float(P.geom_weight);
-  tw.sg = 1.0f;
-  tw.wf = float(P.feat_weight);
-  tw.sf = -1.0f;
+  const bool g = P.use_geom && P.geom_weight != 0.0;
+  tw.b = g ? P.geom_weight : P.feat_weight;
+  const double rho = tw.b != 0.0 ? P.feat_weight / tw.b : 0.0;
+  tw.fdiv = float(rho * double(1.0f / (2.0f * kFeatStep)));
+  tw.gon = g ? 1 : 0;
   tw.fthr2 = P.feat_residual_thresh * P.feat_residual_thresh;
   tw.gw = P.geom_weight;
   tw.fw = P.feat_weight;
@@ -154,7 +164,11 @@ __device__ inline uint8_t res_u8(__amdgpu_buffer_rsrc_t r, int voff) {
 template <typename S>
 __device__ inline V3<float> res_ld3(__amdgpu_buffer_rsrc_t r, int i) {
   constexpr int es = int(sizeof(S));
-  const int o = 3 * es * i;
+  // 3 * es * i as a shift and a shift-add: an opaque copy of 2i keeps the
+  // compiler from folding it back into a quarter-rate 32-bit multiply.
+  int i2 = i << 1;
+  asm("" : "+v"(i2));
+  const int o = (i2 + i) * es;
   return V3<float>{res_ld<S>(r, o, 0), res_ld<S>(r, o + es, 0),
                    res_ld<S>(r, o + 2 * es, 0)};
 }
@@ -199,10 +213,10 @@ __device__ inline void corresp_pixel_at(const LevelView<S>& L,
   float u, v;
   project(L.fx, L.fy, L.cx, L.cy, p, u, v);
   int ui, vi;
-  const bool in_u = round_in_range(u, L.W, ui);
-  const bool in_v = round_in_range(v, L.H, vi);
+  const bool in_u = round_px(u, L.W, ui);
+  const bool in_v = round_px(v, L.H, vi);
   if (!valid_src || !in_u || !in_v) return;
-  const int pos = vi * L.W + ui;
+  const int pos = mad24(vi, L.W, ui);
   const uint8_t valid = res_u8(R.tm, pos);
   const V3<float> q = res_ld3<S>(R.tp, pos);
   const V3<float> qn = res_ld3<S>(R.tn, pos);
@@ -236,10 +250,10 @@ __device__ inline void corresp_pixel_at_runs(
   float u, v;
   project(L.fx, L.fy, L.cx, L.cy, p, u, v);
   int ui, vi;
-  const bool in_u = round_in_range(u, L.W, ui);
-  const bool in_v = round_in_range(v, L.H, vi);
+  const bool in_u = round_px(u, L.W, ui);
+  const bool in_v = round_px(v, L.H, vi);
   const bool inside = valid_src && in_u && in_v;
-  const int pos = inside ? vi * L.W + ui : 0;
+  const int pos = inside ? mad24(vi, L.W, ui) : 0;
   const uint8_t valid = res_u8(R.tm, pos);
   const V3<float> q = res_ld3<S>(R.tp, pos);
   const V3<float> qn = res_ld3<S>(R.tn, pos);
@@ -296,22 +310,28 @@ __device__ inline void corresp_pixel(const LevelView<S>& L, const LevelRes& R,
                       cstar);
 }
 
-// Adds one weighted term J' = w J, r' (already scaled) to the JtJ/Jr slots.
-__device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float w,
-                                float r) {
-  float jw[6];
-#pragma unroll
-  for (int a = 0; a < 6; ++a) jw[a] = j[a] * w;
+// Adds one term J, r to the JtJ/Jr slots (the weights enter with the block
+// partials). BORN: the slots start from this term (the first term of a
+// lane's first pixel), so they need no zeroing and no add to 0.
+template <bool BORN>
+__device__ inline void acc_term(float (&acc)[kSlots], const float j[6], float r) {
   int k = 0;
 #pragma unroll
   for (int a = 0; a < 6; ++a)
 #pragma unroll
     for (int b = a; b < 6; ++b) {
-      acc[k] = fmaf(jw[a], jw[b], acc[k]);
+      acc[k] = BORN ? j[a] * j[b] : fmaf(j[a], j[b], acc[k]);
       ++k;
     }
 #pragma unroll
-  for (int a = 0; a < 6; ++a) acc[21 + a] = fmaf(jw[a], r, acc[21 + a]);
+  for (int a = 0; a < 6; ++a)
+    acc[21 + a] = BORN ? j[a] * r : fmaf(j[a], r, acc[21 + a]);
+}
+
+// acc[k] + x, or x for a slot that starts here.
+template <bool BORN>
+__device__ inline float acc_add(float a, float x) {
+  return BORN ? x : a + x;
 }
 
 // The four bilinear taps of one sample from the resource of one [H, W]
@@ -320,7 +340,7 @@ template <typename S>
 __device__ inline void res_taps(__amdgpu_buffer_rsrc_t r, int W,
                                 const Bil<float>& b, float t[4]) {
   constexpr int es = int(sizeof(S));
-  const int o = (b.vi * W + b.ui) * es;
+  const int o = mad24(b.vi, W, b.ui) * es;
   const int d = o + W * es;
   t[0] = res_ld<S>(r, o, 0);
   t[1] = res_ld<S>(r, o + es, 0);
@@ -373,7 +393,7 @@ __device__ inline void res_pair16(__amdgpu_buffer_rsrc_t r, int o, int last8,
 template <typename S>
 __device__ inline void res_taps_pair(__amdgpu_buffer_rsrc_t r, int W, int last8,
                                      const Bil<float>& b, float t[4]) {
-  const int o = (b.vi * W + b.ui) * 2;
+  const int o = mad24(b.vi, W, b.ui) * 2;
   res_pair16<S>(r, o, last8, t[0], t[1]);
   res_pair16<S>(r, o + W * 2, last8, t[2], t[3]);
 }
@@ -409,12 +429,16 @@ __device__ inline float bil_mix_res(int W, const Bil<float>& b, const float t[4]
 }
 
 // feature_terms<float, S, S, true, NB> (icp_math.hpp) with its taps and
-// source values read through buffer resources: the same values in the same
-// arithmetic, so the same bits.
+// source values read through buffer resources, the same samples in the same
+// arithmetic. (The finite differences subtract samples 0.1 px apart, so a
+// last-bit change in a sample moves the derivative by ~1e-5 of its value:
+// the samples keep the reference's operation order.) The derivative divisor
+// carries the feature weight ratio (TermWeights::fdiv), so J comes out as
+// rho J.
 template <typename S, int NB>
 __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
                                           const V3<float>& p, float u, float v,
-                                          float j[6], float& rlast) {
+                                          float div, float j[6], float& rlast) {
 #pragma clang fp contract(off)
   constexpr int es = int(sizeof(S));
   // The source value of channel c is element c * n + idx of one resource
@@ -423,8 +447,7 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
   const int sch = int(L.n) * es;  // bytes per source channel
   const int so = int(idx) * es;
   const int H = L.H, W = L.W, C = L.C;
-  const float h = 0.05f;
-  const float div = 1.0f / (2.0f * h);
+  const float h = kFeatStep;
   const Bil<float> c0 = make_bil(u, v);
   const Bil<float> bu0 = make_bil(u + h, v);
   const Bil<float> bu1 = make_bil(fmax(u - h, 0.0f), v);
@@ -478,22 +501,20 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
     }
   }
   const float res = last - flast;
+  // J (values only, no decisions): FMA forms.
   const float du = (su0 - su1) * div;
   const float dv = (sv0 - sv1) * div;
   const float iz = __builtin_amdgcn_rcpf(p.z);
-  const float iz2 = iz * iz;
-  const float j00 = L.fx * iz;
-  const float j02 = -p.x * L.fx * iz2;
-  const float j11 = L.fy * iz;
-  const float j12 = -p.y * L.fy * iz2;
-  const V3<float> g{du * j00, dv * j11, du * j02 + dv * j12};
-  const V3<float> c = cross3(p, g);
-  j[0] = g.x;
-  j[1] = g.y;
-  j[2] = g.z;
-  j[3] = c.x;
-  j[4] = c.y;
-  j[5] = c.z;
+  const float fxz = L.fx * iz, fyz = L.fy * iz;
+  const float gx = du * fxz, gy = dv * fyz;
+  // J of the projection: g = (du fx/z, dv fy/z, -(du fx x + dv fy y)/z^2).
+  const float gz = -fmaf(gx, p.x, gy * p.y) * iz;
+  j[0] = gx;
+  j[1] = gy;
+  j[2] = gz;
+  j[3] = fmaf(p.y, gz, -(p.z * gy));
+  j[4] = fmaf(p.z, gx, -(p.x * gz));
+  j[5] = fmaf(p.x, gy, -(p.y * gx));
   rlast = res;
   sq += res * res;
   return sq;
@@ -501,70 +522,96 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
 
 // Point-to-plane and feature terms of source pixel idx when the map names it
 // as its target pixel's owner (src/icp_jacobian_geometric.cu:54-85,
-// src/icp_jacobian_feature.cu:58-109). The owner word, the target point and
-// normal and the feature taps around (u, v) are loaded whatever the owner
-// test says and the terms are accumulated through selects, so all of those
-// loads issue in one batch: a pixel costs two memory round trips. A pixel
-// that does not own its target adds exact zeros (selected, never multiplied
-// in, so NaN or inf in its unused data cannot leak into the sums).
-template <typename S, bool GEOM, bool FEAT, int NB = 4>
+// src/icp_jacobian_feature.cu:58-109). The code has no branch: a pixel that
+// is not valid, projects outside the target or does not own its target
+// pixel reads target pixel 0 (or its clamped feature cell) and adds exact
+// zeros (selected, never multiplied in, so NaN or inf in its unused data
+// cannot leak into the sums). The owner word, the target point and normal
+// and the feature taps are loaded whatever the owner test says, so all of
+// them issue in one batch: a pixel costs two memory round trips. The
+// projection, the rounding and the owner test keep the reference's
+// arithmetic (the same decisions as the correspondence pass); the values of
+// J and r use FMA. BORN: this is the lane's first pixel, the slots start
+// from its terms.
+template <typename S, bool GEOM, bool FEAT, bool BORN, int NB = 4>
 __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
                                          const LevelRes& R, const float* m,
                                          const unsigned long long* __restrict__ map,
-                                         uint32_t tag, int64_t idx,
-                                         uint8_t valid_src, V3<float> ps,
-                                         const TermWeights& tw,
+                                         uint32_t tag, int64_t idx, bool valid_src,
+                                         V3<float> ps, const TermWeights& tw,
                                          float (&acc)[kSlots]) {
-#pragma clang fp contract(off)
-  const V3<float> p = rigid_point(m, ps);
+  V3<float> p;
   float u, v;
-  project(L.fx, L.fy, L.cx, L.cy, p, u, v);
+  {
+#pragma clang fp contract(off)
+    p = rigid_point(m, ps);
+    project(L.fx, L.fy, L.cx, L.cy, p, u, v);
+  }
   int ui, vi;
-  const bool in_u = round_in_range(u, L.W, ui);
-  const bool in_v = round_in_range(v, L.H, vi);
-  if (!valid_src || !in_u || !in_v) return;
-  const int pos = vi * L.W + ui;
+  const bool in_u = round_px(u, L.W, ui);
+  const bool in_v = round_px(v, L.H, vi);
+  const bool inside = valid_src && in_u && in_v;
+  const int pos = inside ? mad24(vi, L.W, ui) : 0;
   const __amdgpu_buffer_rsrc_t rmap = make_res(map, L.HW);
   const uint32_t klo = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos, 0, 0);
   const uint32_t khi = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos + 4, 0, 0);
-  const bool own = key_owned((uint64_t(khi) << 32) | klo, uint32_t(idx), tag);
+  // The owner and use tests combine with `&`, never `&&`: a short-circuit
+  // is a branch, and the compiler sinks into it every load whose value only
+  // the owner needs (the target point, the feature taps), which then wait
+  // for the key -- a second round trip.
+  const bool own =
+      inside & key_owned((uint64_t(khi) << 32) | klo, uint32_t(idx), tag);
   if (GEOM) {
     const V3<float> q = res_ld3<S>(R.tp, pos);
     const V3<float> qn = res_ld3<S>(R.tn, pos);
-    const float r = dot3(sub3(q, p), qn);
-    const V3<float> c = cross3(p, qn);
-    const float j[6] = {own ? qn.x : 0.0f, own ? qn.y : 0.0f, own ? qn.z : 0.0f,
-                        own ? c.x : 0.0f,  own ? c.y : 0.0f,  own ? c.z : 0.0f};
+    const V3<float> d = sub3(q, p);
+    const float r = fmaf(d.x, qn.x, fmaf(d.y, qn.y, d.z * qn.z));
+    const bool jon = own & (tw.gon != 0);
+    const float j[6] = {jon ? qn.x : 0.0f,
+                        jon ? qn.y : 0.0f,
+                        jon ? qn.z : 0.0f,
+                        jon ? fmaf(p.y, qn.z, -(p.z * qn.y)) : 0.0f,
+                        jon ? fmaf(p.z, qn.x, -(p.x * qn.z)) : 0.0f,
+                        jon ? fmaf(p.x, qn.y, -(p.y * qn.x)) : 0.0f};
     const float rs = own ? r : 0.0f;
-    acc_term(acc, j, tw.wg, rs * tw.sg);
-    acc[27] = fmaf(rs, rs, acc[27]);
-    acc[29] += own ? 1.0f : 0.0f;
+    acc_term<BORN>(acc, j, rs);
+    acc[27] = BORN ? rs * rs : fmaf(rs, rs, acc[27]);
+    acc[29] = acc_add<BORN>(acc[29], own ? 1.0f : 0.0f);
   }
   if (FEAT) {
+    constexpr bool FB = BORN && !GEOM;
+    // Lanes outside the target sample cell (0, 0): in range, unused.
+    const float uu = inside ? u : 0.0f, vv = inside ? v : 0.0f;
     float jf[6], rl;
-    const float sq = feature_terms_res<S, NB>(L, idx, p, u, v, jf, rl);
-    const bool use = own && !(sq > tw.fthr2);
+    const float sq = feature_terms_res<S, NB>(L, idx, p, uu, vv, tw.fdiv, jf, rl);
+    const bool use = own & !(sq > tw.fthr2);
     float j[6];
 #pragma unroll
     for (int a = 0; a < 6; ++a) j[a] = use ? jf[a] : 0.0f;
-    acc_term(acc, j, tw.wf, (use ? rl : 0.0f) * tw.sf);
-    acc[28] += use ? sq : 0.0f;
-    acc[30] += use ? 1.0f : 0.0f;
+    acc_term<FB>(acc, j, use ? -rl : 0.0f);
+    acc[28] = acc_add<FB>(acc[28], use ? sq : 0.0f);
+    acc[30] = acc_add<FB>(acc[30], use ? 1.0f : 0.0f);
+  }
+  if (BORN) {
+    if (!GEOM) acc[27] = acc[29] = 0.0f;
+    if (!FEAT) acc[28] = acc[30] = 0.0f;
+    acc[31] = 0.0f;
   }
 }
 
 // jacobian_pixel_at with the source pixel's mask and point loaded here.
-template <typename S, bool GEOM, bool FEAT, int NB = 4>
+template <typename S, bool GEOM, bool FEAT, bool BORN, int NB = 4>
 __device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
                                       const float* m,
                                       const unsigned long long* __restrict__ map,
-                                      uint32_t tag, int64_t idx,
+                                      uint32_t tag, int64_t idx, bool in_run,
                                       const TermWeights& tw,
                                       float (&acc)[kSlots]) {
   const uint8_t valid_src = res_u8(R.sm, int(idx));
   const V3<float> ps = res_ld3<S>(R.sp, int(idx));
-  jacobian_pixel_at<S, GEOM, FEAT, NB>(L, R, m, map, tag, idx, valid_src, ps,
-                                       tw, acc);
+  jacobian_pixel_at<S, GEOM, FEAT, BORN, NB>(L, R, m, map, tag, idx,
+                                             in_run & (valid_src != 0), ps, tw,
+                                             acc);
 }
 
 // G^T G of source pixel idx from its matched target point
@@ -669,14 +716,19 @@ __device__ inline void wave_reduce_slots(float (&v)[kSlots], int lane) {
 // Reduces the block's slots and stores them in FP64 at `out` (32 doubles),
 // with no fence: every consumer runs in a later launch (a fence and a ticket
 // for a consumer in the same launch cost each block ~20 ns at the memory
-// side). `sh` is LDS of BLOCK / 64 * kSlots doubles (kShared for kBlock).
+// side). The wavefront sums enter FP64 scaled by the weight the per-pixel
+// sums left out (TermWeights): b^2 for the 21 JtJ slots, b for the 6 Jr
+// slots, 1 for the bookkeeping (b = 1: the slots as accumulated). `sh` is
+// LDS of BLOCK / 64 * kSlots doubles (kShared for kBlock).
 template <int BLOCK = kBlock>
 __device__ inline void block_partial(float (&acc)[kSlots], double* out,
-                                     double* sh) {
+                                     double* sh, double b = 1.0) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   wave_reduce_slots(acc, lane);
-  if ((lane & 1) == 0) sh[wave * kSlots + (lane >> 1)] = double(acc[0]);
+  const int slot = lane >> 1;
+  const double scale = slot < 21 ? b * b : (slot < 27 ? b : 1.0);
+  if ((lane & 1) == 0) sh[wave * kSlots + slot] = double(acc[0]) * scale;
   __syncthreads();
   if (threadIdx.x < kSlots) {
     double s = 0.0;

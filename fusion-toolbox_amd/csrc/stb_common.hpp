@@ -141,6 +141,27 @@ __device__ inline bool round_in_range(T u, int limit, int& out) {
   return true;
 }
 
+// round_in_range for FP32 without a branch or a rounding instruction:
+// round(u) (half away from zero) lies in [0, limit) exactly when
+// -0.5 < u < limit - 0.5 (NaN fails both), and for such u, with t the
+// truncation of u and f = u - t (exact), round(u) = t + (f >= 0.5); for u in
+// (-0.5, 0) that is t = 0, f < 0, giving 0 like the -0.0 that round()
+// returns. `out` is set whatever the verdict and is meaningful only when it
+// is true, so callers use it under a select.
+__device__ inline bool round_px(float u, int limit, int& out) {
+  const int t = int(u);  // one truncating conversion
+  const float f = u - float(t);
+  out = t + (f >= 0.5f ? 1 : 0);
+  return u > -0.5f && u < float(limit) - 0.5f;
+}
+
+// a * b + c for 0 <= a, b < 2^24 whose product fits 32 bits (a full-rate
+// 24-bit multiply-add instead of a quarter-rate 32-bit one): pixel offsets of
+// levels whose arrays hold fewer than 2^31 bytes.
+__device__ inline int mad24(int a, int b, int c) {
+  return __umul24(uint32_t(a), uint32_t(b)) + c;
+}
+
 template <typename T>
 __device__ inline int trunc_int(T x) {
   return int(x);  // callers keep x inside [-1, limit)
