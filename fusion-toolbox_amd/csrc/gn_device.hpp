@@ -412,6 +412,83 @@ __device__ inline void res_taps_any(__amdgpu_buffer_rsrc_t r, int W, bool pairs,
   res_taps<S>(r, W, b, t);
 }
 
+// FP32 taps with wide loads. Buffer loads check the range per dword (a
+// dword past the resource's end reads 0, the dwords before it are read),
+// so a 2- or 4-texel load near the end of a channel keeps bil_mix_res's
+// zero padding.
+__device__ inline void res_ld_f2(__amdgpu_buffer_rsrc_t r, int o, float& a,
+                                 float& b) {
+  const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, o, 0, 0);
+  a = __uint_as_float(w[0]);
+  b = __uint_as_float(w[1]);
+}
+
+// res_taps<float> with each row's two taps in one 8-byte load.
+__device__ inline void res_taps_f32(__amdgpu_buffer_rsrc_t r, int W,
+                                    const Bil<float>& b, float t[4]) {
+  const int o = mad24(b.vi, W, b.ui) * 4;
+  res_ld_f2(r, o, t[0], t[1]);
+  res_ld_f2(r, o + W * 4, t[2], t[3]);
+}
+
+// The taps of the last channel's five samples (c0 at (u, v), bu0/bu1 at
+// u +- h, bv0/bv1 at v +- h) from four loads instead of ten. The u samples'
+// cells lie one column left of, at, or one column right of c0's, so two
+// 16-byte loads of rows fv and fv + 1 from column fu - 1 (from column 0
+// when fu = 0, where no sample reaches left) cover them; the v samples'
+// cells one row above or below, so rows fv - 1 and fv + 2 at columns fu,
+// fu + 1 add two 8-byte loads (read only when a sample's cell moved there;
+// past the channel's start or end they read 0). Each sample then selects
+// its taps from registers.
+__device__ inline void res_taps5_f32(__amdgpu_buffer_rsrc_t r, int W,
+                                     const Bil<float>& c0, const Bil<float>& bu0,
+                                     const Bil<float>& bu1, const Bil<float>& bv0,
+                                     const Bil<float>& bv1, float t0[4],
+                                     float t1[4], float t2[4], float t3[4],
+                                     float t4[4]) {
+  const int o = mad24(c0.vi, W, c0.ui) * 4;  // byte offset of (fu, fv)
+  const bool left = c0.ui >= 1;
+  const int ow = left ? o - 4 : o;  // the 4-texel window's first column
+  const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, ow, 0, 0);
+  const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, ow + W * 4, 0, 0);
+  float up0, up1, dn0, dn1;
+  res_ld_f2(r, o - W * 4, up0, up1);
+  res_ld_f2(r, o + W * 8, dn0, dn1);
+  // Rows fv (A) and fv + 1 (B) at columns fu - 1 .. fu + 2.
+  float A[4], B[4];
+  A[0] = __uint_as_float(a[0]);
+  B[0] = __uint_as_float(b[0]);
+#pragma unroll
+  for (int k = 1; k < 4; ++k) {
+    A[k] = __uint_as_float(left ? a[k] : a[k - 1]);
+    B[k] = __uint_as_float(left ? b[k] : b[k - 1]);
+  }
+  t0[0] = A[1];
+  t0[1] = A[2];
+  t0[2] = B[1];
+  t0[3] = B[2];
+  const bool ru = bu0.ui != c0.ui;  // bu0 one column right
+  t1[0] = ru ? A[2] : A[1];
+  t1[1] = ru ? A[3] : A[2];
+  t1[2] = ru ? B[2] : B[1];
+  t1[3] = ru ? B[3] : B[2];
+  const bool lu = bu1.ui != c0.ui;  // bu1 one column left
+  t2[0] = lu ? A[0] : A[1];
+  t2[1] = lu ? A[1] : A[2];
+  t2[2] = lu ? B[0] : B[1];
+  t2[3] = lu ? B[1] : B[2];
+  const bool dv = bv0.vi != c0.vi;  // bv0 one row down
+  t3[0] = dv ? B[1] : A[1];
+  t3[1] = dv ? B[2] : A[2];
+  t3[2] = dv ? dn0 : B[1];
+  t3[3] = dv ? dn1 : B[2];
+  const bool uv = bv1.vi != c0.vi;  // bv1 one row up
+  t4[0] = uv ? up0 : A[1];
+  t4[1] = uv ? up1 : A[2];
+  t4[2] = uv ? A[1] : B[1];
+  t4[3] = uv ? A[2] : B[2];
+}
+
 // bil_mix for taps read through a resource over one channel: the taps of
 // the row below the last lie past the resource's end and read 0, which is
 // the value bil_mix selects for them, so only the right-edge select remains
@@ -468,11 +545,15 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
                      L.HW >= 4;
   const int last8 = int(L.HW) * 2 - 8;
   float t0[4], t1[4], t2[4], t3[4], t4[4];
-  res_taps_any<S>(rl, W, pairs, last8, c0, t0);
-  res_taps_any<S>(rl, W, pairs, last8, bu0, t1);
-  res_taps_any<S>(rl, W, pairs, last8, bu1, t2);
-  res_taps_any<S>(rl, W, pairs, last8, bv0, t3);
-  res_taps_any<S>(rl, W, pairs, last8, bv1, t4);
+  if constexpr (std::is_same<S, float>::value) {
+    res_taps5_f32(rl, W, c0, bu0, bu1, bv0, bv1, t0, t1, t2, t3, t4);
+  } else {
+    res_taps_any<S>(rl, W, pairs, last8, c0, t0);
+    res_taps_any<S>(rl, W, pairs, last8, bu0, t1);
+    res_taps_any<S>(rl, W, pairs, last8, bu1, t2);
+    res_taps_any<S>(rl, W, pairs, last8, bv0, t3);
+    res_taps_any<S>(rl, W, pairs, last8, bv1, t4);
+  }
   const float flast = res_ld<S>(rs, so + (C - 1) * sch, 0);
   // Row pairs past the channel's end read shifted data, not 0: BF16 keeps
   // both selects.
@@ -490,8 +571,11 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
       const int c = c0i + k < last_c ? c0i + k : last_c;
-      res_taps_any<S>(make_res(L.tf + int64_t(c) * L.HW, L.HW), W, pairs, last8,
-                      c0, taps[k]);
+      const __amdgpu_buffer_rsrc_t rc = make_res(L.tf + int64_t(c) * L.HW, L.HW);
+      if constexpr (std::is_same<S, float>::value)
+        res_taps_f32(rc, W, c0, taps[k]);
+      else
+        res_taps_any<S>(rc, W, pairs, last8, c0, taps[k]);
       fs[k] = res_ld<S>(rs, so + c * sch, 0);
     }
 #pragma unroll
