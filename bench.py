@@ -148,7 +148,7 @@ def prepare(seed, dev):
     return pair, ms, gpu_pair_levels(ms, pair, dev)
 
 
-def gpu_pair_levels(ms, pair, dev, storage=None):
+def gpu_pair_levels(ms, pair, dev, storage=None, feat_layout="planar"):
     """Bilateral-filtered frame point clouds and pyramid levels of a pair."""
     import torch
     from slamtb.camera import KCamera
@@ -162,7 +162,8 @@ def gpu_pair_levels(ms, pair, dev, storage=None):
         fps.append(FramePointCloud.from_frame(
             Frame(FrameInfo(kcam, 1.0, 0.0), depth_image=depth), device=dev))
     feats = [torch.from_numpy(pair[f"{s}_feats"]).to(dev) for s in ("src", "tgt")]
-    return ms.levels(fps[0], fps[1], feats[0], feats[1], storage=storage)
+    return ms.levels(fps[0], fps[1], feats[0], feats[1], storage=storage,
+                     feat_layout=feat_layout)
 
 
 def capture(fn):
@@ -577,8 +578,11 @@ def batched_oracle_check(batch, pairs):
 def bf16_section(seed, dev, with_oracle):
     """1280x720, 4 levels (10/5/3/2 it), geometric + 32-channel unit
     descriptors; BF16 and FP16 storage against FP32 storage of the same
-    scene: iterations per second at the top level, its correspondence and
-    Jacobian launches alone, the whole registration, and pose errors."""
+    scene, the 16-bit ones also with channel-interleaved features (the
+    same results bit for bit): iterations per second at the top level, its
+    correspondence and Jacobian launches alone, the whole registration, and
+    pose errors."""
+    import numpy as np
     import torch
     from slamtb import synthetic, metrics
     from slamtb.registration import ICPOdometry, MultiscaleRegistration
@@ -593,9 +597,13 @@ def bf16_section(seed, dev, with_oracle):
                              kcam=synthetic.intrinsics(1280, 720), channels=32,
                              descriptor=True)
     out, poses = {}, {}
-    for name, storage in (("bf16", torch.bfloat16), ("fp16", torch.float16),
-                          ("fp32", torch.float32)):
-        levels = gpu_pair_levels(ms, pr, dev, storage=storage)
+    for name, storage, layout in (
+            ("bf16", torch.bfloat16, "planar"),
+            ("bf16_interleaved", torch.bfloat16, "interleaved"),
+            ("fp16", torch.float16, "planar"),
+            ("fp16_interleaved", torch.float16, "interleaved"),
+            ("fp32", torch.float32, "planar")):
+        levels = gpu_pair_levels(ms, pr, dev, storage=storage, feat_layout=layout)
         est, top = levels[-1]
         params = est.params(True)
         run = GNRunner(dev)
@@ -623,7 +631,7 @@ def bf16_section(seed, dev, with_oracle):
                  "matched_top_level": matched,
                  "registration_ms": t_reg * 1e3,
                  "pose_err_vs_gt": list(metrics.pose_error(pr["T_gt"], res.transform))}
-        if with_oracle and name != "fp32":
+        if with_oracle and layout == "planar" and name != "fp32":
             ref = oracle_levels([(e, L) for e, L in levels])
             entry["pose_err_vs_oracle"] = list(metrics.pose_error(ref["transform"],
                                                                   res.transform))
@@ -632,6 +640,10 @@ def bf16_section(seed, dev, with_oracle):
     for name in ("bf16", "fp16"):
         out[f"pose_diff_{name}_vs_fp32"] = list(metrics.pose_error(poses["fp32"],
                                                                    poses[name]))
+        out[f"{name}_interleaved_equals_planar"] = bool(np.array_equal(
+            np.asarray(poses[name]), np.asarray(poses[f"{name}_interleaved"])))
+    best16 = max(out["bf16"]["iters_per_s"], out["bf16_interleaved"]["iters_per_s"])
+    out["bf16_best_over_fp32"] = best16 / out["fp32"]["iters_per_s"]
     return out
 
 

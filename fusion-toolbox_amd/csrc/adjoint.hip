@@ -553,6 +553,8 @@ extern "C" int stb_icp_batch_adjoint(const void* plan,
               "stb_icp_batch_adjoint: NULL argument");
   STB_REQUIRE(sizes->storage == STB_F32,
               "stb_icp_batch_adjoint: FP32 storage only");
+  STB_REQUIRE(sizes->feat_layout == STB_FEAT_PLANAR,
+              "stb_icp_batch_adjoint: channel-planar features only");
   STB_REQUIRE(params->use_geom || params->use_feat,
               "stb_icp_batch_adjoint: no active term");
   const Scratch sc = carve_scratch(scratch, sizes->num_pairs,
@@ -584,6 +586,8 @@ extern "C" int stb_icp_batch_system64(const void* plan,
               "stb_icp_batch_system64: NULL argument");
   STB_REQUIRE(sizes->storage == STB_F32,
               "stb_icp_batch_system64: FP32 storage only");
+  STB_REQUIRE(sizes->feat_layout == STB_FEAT_PLANAR,
+              "stb_icp_batch_system64: channel-planar features only");
   STB_REQUIRE(params->use_geom || params->use_feat,
               "stb_icp_batch_system64: no active term");
   const Scratch sc = carve_scratch(scratch, sizes->num_pairs,

@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-template <typename S, bool GEOM, bool FEAT, int NB>
+template <typename S, bool GEOM, bool FEAT, int NB, bool IL = false>
 __global__ void __launch_bounds__(kBlock, 4)
     batch_jacobian_kernel(const void* __restrict__ plan,
                           stb_icp_state* __restrict__ states,
@@ -113,8 +113,8 @@ __global__ void __launch_bounds__(kBlock, 4)
       const uint8_t vs_n = res_u8(R.sm, int(nxt));
       const V3<float> ps_n = res_ld3<S>(R.sp, int(nxt));
       const bool ok = (int(threadIdx.x) < pv.tile_px) & (idx < L.n) & (vs != 0);
-      jacobian_pixel_at<S, GEOM, FEAT, true, NB>(L, R, m, map, tag, idx, ok,
-                                                 ps, tw, acc);
+      jacobian_pixel_at<S, GEOM, FEAT, true, NB, IL>(L, R, m, map, tag, idx,
+                                                     ok, ps, tw, acc);
       idx = nxt;
       vs = vs_n;
       ps = ps_n;
@@ -124,8 +124,9 @@ __global__ void __launch_bounds__(kBlock, 4)
       const uint8_t vs_n = res_u8(R.sm, int(nxt));
       const V3<float> ps_n = res_ld3<S>(R.sp, int(nxt));
       if (idx < L.n)
-        jacobian_pixel_at<S, GEOM, FEAT, false, NB>(L, R, m, map, tag, idx,
-                                                    vs != 0, ps, tw, acc);
+        jacobian_pixel_at<S, GEOM, FEAT, false, NB, IL>(L, R, m, map, tag,
+                                                        idx, vs != 0, ps, tw,
+                                                        acc);
       idx = nxt;
       vs = vs_n;
       ps = ps_n;
@@ -237,11 +238,16 @@ int grid_tiles(const stb_icp_params* P, int tiles, int dflt) {
 
 template <typename S, bool GEOM, bool FEAT>
 void launch_batch_jacobian(const void* plan, int num_pairs, int max_channels,
-                           stb_icp_state* states, const Scratch& sc, int grid,
-                           const TermWeights& tw, int do_solve, int fixed,
-                           hipStream_t s) {
-  // Batches of two feature channels when no pair has four besides its last.
-  if (FEAT && max_channels - 1 < 4)
+                           int feat_layout, stb_icp_state* states,
+                           const Scratch& sc, int grid, const TermWeights& tw,
+                           int do_solve, int fixed, hipStream_t s) {
+  // Batches of two feature channels when no pair has four besides its last;
+  // interleaved features in chunks of eight.
+  if (FEAT && feat_layout == STB_FEAT_INTERLEAVED)
+    hipLaunchKernelGGL((batch_jacobian_kernel<S, GEOM, FEAT, 4, true>),
+                       dim3(grid), dim3(kBlock), 0, s, plan, states, sc.map0,
+                       sc.map1, sc.partials, tw, fixed);
+  else if (FEAT && max_channels - 1 < 4)
     hipLaunchKernelGGL((batch_jacobian_kernel<S, GEOM, FEAT, 2>), dim3(grid),
                        dim3(kBlock), 0, s, plan, states, sc.map0, sc.map1,
                        sc.partials, tw, fixed);
@@ -292,14 +298,17 @@ void launch_batch(const void* plan, const stb_icp_batch_sizes& z,
     }
     const int solve = phase == kIterate ? 1 : 0;
     if (P.use_geom && P.use_feat)
-      launch_batch_jacobian<S, true, true>(plan, z.num_pairs, z.max_channels, states, sc, gj,
-                                           tw, solve, fixed, s);
+      launch_batch_jacobian<S, true, true>(plan, z.num_pairs, z.max_channels,
+                                           z.feat_layout, states, sc, gj, tw,
+                                           solve, fixed, s);
     else if (P.use_geom)
-      launch_batch_jacobian<S, true, false>(plan, z.num_pairs, z.max_channels, states, sc, gj,
-                                            tw, solve, fixed, s);
+      launch_batch_jacobian<S, true, false>(plan, z.num_pairs, z.max_channels,
+                                            z.feat_layout, states, sc, gj, tw,
+                                            solve, fixed, s);
     else
-      launch_batch_jacobian<S, false, true>(plan, z.num_pairs, z.max_channels, states, sc, gj,
-                                            tw, solve, fixed, s);
+      launch_batch_jacobian<S, false, true>(plan, z.num_pairs, z.max_channels,
+                                            z.feat_layout, states, sc, gj, tw,
+                                            solve, fixed, s);
   }
 }
 
@@ -361,6 +370,7 @@ extern "C" int stb_icp_batch_plan(int32_t num_pairs,
               "stb_icp_batch_plan: tile_px must be a positive multiple of %d",
               kBlock);
   const int32_t storage = levels[0].storage;
+  const int32_t layout = levels[0].feat_layout;
   std::vector<int32_t> start(size_t(num_pairs) + 1, 0);
   std::vector<int64_t> off(size_t(num_pairs), 0);
   int64_t tiles = 0, maps = 0;
@@ -378,6 +388,14 @@ extern "C" int stb_icp_batch_plan(int32_t num_pairs,
                 p, (long long)kMaxKeyIndex);
     STB_REQUIRE(L.storage == storage,
                 "stb_icp_batch_plan: all pairs must share one storage type");
+    STB_REQUIRE(L.feat_layout == layout,
+                "stb_icp_batch_plan: all pairs must share one feature layout");
+    STB_REQUIRE(layout == STB_FEAT_PLANAR ||
+                    (layout == STB_FEAT_INTERLEAVED && storage != STB_F32 &&
+                     L.channels <= 32 && L.channels % 2 == 0),
+                "stb_icp_batch_plan: pair %d: feat_layout must be planar, or "
+                "interleaved with 16-bit storage and an even channel count "
+                "of at most 32", p);
     STB_REQUIRE(level_addressable(L),
                 "stb_icp_batch_plan: every array of pair %d must hold fewer "
                 "than 2^31 bytes", p);
@@ -398,7 +416,7 @@ extern "C" int stb_icp_batch_plan(int32_t num_pairs,
   sizes->plan_bytes = l.bytes;
   sizes->scratch_bytes = sc.bytes;
   sizes->max_channels = max_channels;
-  sizes->reserved = 0;
+  sizes->feat_layout = layout;
   if (plan_host == nullptr) return STB_OK;  // size query
   STB_REQUIRE(plan_capacity >= l.bytes, "stb_icp_batch_plan: buffer too small");
   char* b = static_cast<char*>(plan_host);

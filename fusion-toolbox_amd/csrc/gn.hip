@@ -178,7 +178,8 @@ __global__ void __launch_bounds__(kSolveBlock)
 // before any read of the state or the record: those were written by the
 // previous launch, miss this XCD's L2, and would otherwise hold the first
 // memory round trip back by one of their own.
-template <typename S, bool GEOM, bool FEAT, bool ONE_PX, int NB, int BLOCK>
+template <typename S, bool GEOM, bool FEAT, bool ONE_PX, int NB, int BLOCK,
+          bool IL = false>
 __global__ void __launch_bounds__(BLOCK)
     jacobian_fused_kernel(stb_icp_level Ld, stb_icp_state* __restrict__ st,
                           const SolveRecord* __restrict__ rec,
@@ -226,15 +227,15 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
   for (int i = 0; i < 12; ++i) m[i] = rt[i];
   float acc[kSlots];
-  jacobian_pixel_at<S, GEOM, FEAT, true, NB>(L, R, m, map, tag, i0,
+  jacobian_pixel_at<S, GEOM, FEAT, true, NB, IL>(L, R, m, map, tag, i0,
                                              (i0 < hi) & (vs0 != 0), ps0, tw,
                                              acc);
   if (!ONE_PX) {
     if (i1 < hi)
-      jacobian_pixel_at<S, GEOM, FEAT, false, NB>(L, R, m, map, tag, i1,
+      jacobian_pixel_at<S, GEOM, FEAT, false, NB, IL>(L, R, m, map, tag, i1,
                                                   vs1 != 0, ps1, tw, acc);
     for (int64_t idx = i1 + BLOCK; idx < hi; idx += BLOCK)
-      jacobian_pixel<S, GEOM, FEAT, false, NB>(L, R, m, map, tag, idx, true,
+      jacobian_pixel<S, GEOM, FEAT, false, NB, IL>(L, R, m, map, tag, idx, true,
                                                tw, acc);
   }
   if (!fixed_map && tag == 0) {
@@ -420,6 +421,16 @@ int check_level(const stb_icp_level* L, bool need_feats) {
     STB_REQUIRE(L->src_feats && L->tgt_feats && L->channels > 0,
                 "feature term needs src_feats, tgt_feats and channels > 0");
   }
+  STB_REQUIRE(L->feat_layout == STB_FEAT_PLANAR ||
+                  L->feat_layout == STB_FEAT_INTERLEAVED,
+              "feat_layout must be STB_FEAT_PLANAR or STB_FEAT_INTERLEAVED");
+  if (L->feat_layout == STB_FEAT_INTERLEAVED && need_feats) {
+    STB_REQUIRE(L->storage != STB_F32 && L->channels <= 32 &&
+                    L->channels % 2 == 0,
+                "interleaved features need 16-bit storage and an even channel "
+                "count of at most 32, got storage %d with %d channels",
+                L->storage, L->channels);
+  }
   return STB_OK;
 }
 
@@ -427,7 +438,7 @@ int check_level(const stb_icp_level* L, bool need_feats) {
 // the loop; kLinearize and kInformation are one-off passes on the spare map.
 enum Phase { kCorrespond, kUpdate, kLinearize, kInformation, kJacobians };
 
-template <typename S, bool GEOM, bool FEAT, int NB>
+template <typename S, bool GEOM, bool FEAT, int NB, bool IL = false>
 void launch_jacobian_nb(const stb_icp_level& L, const JacShape& js,
                         const Workspace& w, const SolveRecord* rec,
                         const TermWeights& tw, int fixed_map,
@@ -435,16 +446,16 @@ void launch_jacobian_nb(const stb_icp_level& L, const JacShape& js,
   const int64_t chunk = (L.n_src + js.grid - 1) / js.grid;
   if (js.block == kSolveBlock)
     hipLaunchKernelGGL(
-        (jacobian_fused_kernel<S, GEOM, FEAT, false, NB, kSolveBlock>),
+        (jacobian_fused_kernel<S, GEOM, FEAT, false, NB, kSolveBlock, IL>),
         dim3(js.grid), dim3(kSolveBlock), 0, stream, L, w.st, rec, w.map[0],
         w.map[1], w.map_len, w.partials, tw, fixed_map, chunk);
   else if (js.one_px)
-    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, true, NB, kBlock>),
+    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, true, NB, kBlock, IL>),
                        dim3(js.grid), dim3(kBlock), 0, stream, L, w.st, rec,
                        w.map[0], w.map[1], w.map_len, w.partials, tw,
                        fixed_map, chunk);
   else
-    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, false, NB, kBlock>),
+    hipLaunchKernelGGL((jacobian_fused_kernel<S, GEOM, FEAT, false, NB, kBlock, IL>),
                        dim3(js.grid), dim3(kBlock), 0, stream, L, w.st, rec,
                        w.map[0], w.map[1], w.map_len, w.partials, tw,
                        fixed_map, chunk);
@@ -461,8 +472,11 @@ void launch_jacobian(const stb_icp_level& L, const stb_icp_params& P,
   const JacShape js = jac_shape(P, L.n_src);
   // Feature channels other than the last load in batches of four, or of two
   // when there are fewer than four of them (three-channel colour: no
-  // padding slots).
-  if (FEAT && L.channels - 1 < 4)
+  // padding slots); interleaved features in chunks of eight.
+  if (FEAT && L.feat_layout == STB_FEAT_INTERLEAVED)
+    launch_jacobian_nb<S, GEOM, FEAT, 4, true>(L, js, w, rec, tw, fixed_map,
+                                               stream);
+  else if (FEAT && L.channels - 1 < 4)
     launch_jacobian_nb<S, GEOM, FEAT, 2>(L, js, w, rec, tw, fixed_map, stream);
   else
     launch_jacobian_nb<S, GEOM, FEAT, 4>(L, js, w, rec, tw, fixed_map, stream);

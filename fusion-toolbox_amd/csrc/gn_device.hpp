@@ -76,6 +76,7 @@ struct LevelView {
   int64_t n, HW;
   int H, W, C;
   float fx, fy, cx, cy;
+  int il;  // features interleaved ([n, C] / [H, W, C])
 };
 
 template <typename S>
@@ -98,6 +99,7 @@ __device__ inline LevelView<S> view_of(const stb_icp_level& L) {
   v.fy = L.fy;
   v.cx = L.cx;
   v.cy = L.cy;
+  v.il = L.feat_layout == STB_FEAT_INTERLEAVED;
   return v;
 }
 
@@ -604,6 +606,191 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
   return sq;
 }
 
+// Interleaved features are read 16 bytes per texel at a time: a chunk of
+// CH = 16 / sizeof(S) consecutive channels (4 in FP32, 8 in 16-bit
+// storage) from one load at byte offset o (a multiple of 4), widened to
+// FP32.
+template <typename S>
+constexpr int kChunkCh = 16 / int(sizeof(S));
+
+// Channel q of a chunk held as four raw dwords (16-bit channels widened
+// when read, so a chunk costs 4 registers per texel whatever the storage).
+template <typename S>
+__device__ inline float chunk_ch(const uint32_t (&w)[4], int q) {
+  if constexpr (std::is_same<S, float>::value)
+    return __uint_as_float(w[q]);
+  else
+    return (q & 1) ? widen_hi<S>(w[q >> 1]) : widen_lo<S>(w[q >> 1]);
+}
+
+// chunk_ch for a channel q known only at run time (uniform): a select
+// chain over the four dwords, then the half.
+template <typename S>
+__device__ inline float chunk_ch_at(const uint32_t (&w)[4], int q) {
+  const int d = std::is_same<S, float>::value ? q : (q >> 1);
+  const uint32_t x = d == 0 ? w[0] : d == 1 ? w[1] : d == 2 ? w[2] : w[3];
+  if constexpr (std::is_same<S, float>::value)
+    return __uint_as_float(x);
+  else
+    return (q & 1) ? widen_hi<S>(x) : widen_lo<S>(x);
+}
+
+__device__ inline void res_ld16(__amdgpu_buffer_rsrc_t r, int o,
+                                uint32_t (&w)[4]) {
+  const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, 0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w[k] = a[k];
+}
+
+// The c0 taps and source values of one chunk of channels, raw.
+struct ChunkIL {
+  uint32_t t[4][4];  // texels (fu, fv), (fu + 1, fv), (fu, fv + 1), (fu + 1, fv + 1)
+  uint32_t s[4];     // source pixel
+};
+
+__device__ inline void load_chunk(__amdgpu_buffer_rsrc_t rt,
+                                  __amdgpu_buffer_rsrc_t rs, int o00, int o01,
+                                  int tb, int so, int co, ChunkIL& k) {
+  res_ld16(rt, o00 + co, k.t[0]);
+  res_ld16(rt, o00 + tb + co, k.t[1]);
+  res_ld16(rt, o01 + co, k.t[2]);
+  res_ld16(rt, o01 + tb + co, k.t[3]);
+  res_ld16(rs, so + co, k.s);
+}
+
+// Adds the chunk's channels cb .. (those below C - 1) of the squared
+// residual at sample c0, in channel order.
+template <typename S>
+__device__ inline void add_chunk(const ChunkIL& k, const Bil<float>& c0, int W,
+                                 int cb, int C, float& sq) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int q = 0; q < kChunkCh<S>; ++q) {
+    const float t[4] = {chunk_ch<S>(k.t[0], q), chunk_ch<S>(k.t[1], q),
+                        chunk_ch<S>(k.t[2], q), chunk_ch<S>(k.t[3], q)};
+    const float res = bil_mix_res(W, c0, t) - chunk_ch<S>(k.s, q);
+    sq += (cb + q < C - 1) ? res * res : 0.0f;
+  }
+}
+
+// feature_terms_res for interleaved features (LevelView::il): the same
+// samples in the same arithmetic. The c0 taps of all channels come in
+// chunks of 16 bytes per texel (8 channels of 16-bit storage, 4 of FP32),
+// two chunks in flight at a time; the last channel's five samples also read
+// the eight other texels of its plus-shaped neighbourhood
+// (res_taps5_f32's selection). At most 32 channels; a texel's bytes a
+// multiple of 4 (checked on the host). Chunks past the last channel re-read
+// the last chunk, unused.
+template <typename S>
+__device__ inline float feature_terms_il(const LevelView<S>& L, int64_t idx,
+                                         const V3<float>& p, float u, float v,
+                                         float div, float j[6], float& rlast) {
+#pragma clang fp contract(off)
+  constexpr int es = int(sizeof(S));
+  const int C = L.C, W = L.W;
+  const int tb = C * es;  // bytes per texel
+  const __amdgpu_buffer_rsrc_t rt = make_res(L.tf, int64_t(C) * L.HW);
+  const __amdgpu_buffer_rsrc_t rs = make_res(L.sf, int64_t(C) * L.n);
+  const float h = kFeatStep;
+  const Bil<float> c0 = make_bil(u, v);
+  const Bil<float> bu0 = make_bil(u + h, v);
+  const Bil<float> bu1 = make_bil(fmax(u - h, 0.0f), v);
+  const Bil<float> bv0 = make_bil(u, v + h);
+  const Bil<float> bv1 = make_bil(u, fmax(v - h, 0.0f));
+  const int o = mad24(c0.vi, W, c0.ui);  // texel (fu, fv)
+  const int so = int(idx) * tb;
+  const int lc = (C - 1) * es;
+  // The last channel at texel index x (a neighbour left of column 0 or
+  // above row 0 reads another texel or 0; it is never selected).
+  auto last_at = [&](int x) { return res_ld<S>(rt, x * tb + lc, 0); };
+  // Two chunks in flight (the scheduling barriers keep the compiler from
+  // hoisting every chunk's loads to the top); chunks past the last channel
+  // re-read the last chunk and are not added. The last chunk ends in k[1]
+  // (NCH is even), where the last channel's c0 taps and source value are
+  // read; its eight other neighbours load with the second-to-last chunk's
+  // turn, so they are in flight for the last two chunks only.
+  constexpr int CH = kChunkCh<S>;
+  constexpr int NCH = 32 / CH;  // chunks of 32 channels
+  const int nch = (C + CH - 1) / CH;
+  const int o00 = o * tb, o01 = (o + W) * tb;
+  ChunkIL k[2];
+  load_chunk(rt, rs, o00, o01, tb, so, 0, k[0]);
+  load_chunk(rt, rs, o00, o01, tb, so, 16 * (nch > 1 ? 1 : 0), k[1]);
+  float sq = 0.0f;
+  float A0 = 0.0f, A3 = 0.0f, B0 = 0.0f, B3 = 0.0f;
+  float up0 = 0.0f, up1 = 0.0f, dn0 = 0.0f, dn1 = 0.0f;
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) {
+    __builtin_amdgcn_sched_barrier(0);
+    if (i < nch) add_chunk<S>(k[i & 1], c0, W, i * CH, C, sq);
+    __builtin_amdgcn_sched_barrier(0);
+    if (i + 2 < NCH)
+      load_chunk(rt, rs, o00, o01, tb, so,
+                 16 * (i + 2 < nch ? i + 2 : nch - 1), k[i & 1]);
+    if (i == NCH - 2) {
+      A0 = last_at(o - 1);
+      A3 = last_at(o + 2);
+      B0 = last_at(o + W - 1);
+      B3 = last_at(o + W + 2);
+      up0 = last_at(o - W);
+      up1 = last_at(o - W + 1);
+      dn0 = last_at(o + 2 * W);
+      dn1 = last_at(o + 2 * W + 1);
+    }
+  }
+  const int ql = (C - 1) - (nch - 1) * CH;  // the last channel in its chunk
+  const float A[4] = {A0, chunk_ch_at<S>(k[1].t[0], ql),
+                      chunk_ch_at<S>(k[1].t[1], ql), A3};
+  const float B[4] = {B0, chunk_ch_at<S>(k[1].t[2], ql),
+                      chunk_ch_at<S>(k[1].t[3], ql), B3};
+  const float flast = chunk_ch_at<S>(k[1].s, ql);
+  float t0[4], t1[4], t2[4], t3[4], t4[4];
+  t0[0] = A[1];
+  t0[1] = A[2];
+  t0[2] = B[1];
+  t0[3] = B[2];
+  const bool ru = bu0.ui != c0.ui;
+  t1[0] = ru ? A[2] : A[1];
+  t1[1] = ru ? A[3] : A[2];
+  t1[2] = ru ? B[2] : B[1];
+  t1[3] = ru ? B[3] : B[2];
+  const bool lu = bu1.ui != c0.ui;
+  t2[0] = lu ? A[0] : A[1];
+  t2[1] = lu ? A[1] : A[2];
+  t2[2] = lu ? B[0] : B[1];
+  t2[3] = lu ? B[1] : B[2];
+  const bool dv = bv0.vi != c0.vi;
+  t3[0] = dv ? B[1] : A[1];
+  t3[1] = dv ? B[2] : A[2];
+  t3[2] = dv ? dn0 : B[1];
+  t3[3] = dv ? dn1 : B[2];
+  const bool uv = bv1.vi != c0.vi;
+  t4[0] = uv ? up0 : A[1];
+  t4[1] = uv ? up1 : A[2];
+  t4[2] = uv ? A[1] : B[1];
+  t4[3] = uv ? A[2] : B[2];
+  const float last = bil_mix_res(W, c0, t0);
+  const float su0 = bil_mix_res(W, bu0, t1), su1 = bil_mix_res(W, bu1, t2);
+  const float sv0 = bil_mix_res(W, bv0, t3), sv1 = bil_mix_res(W, bv1, t4);
+  const float res = last - flast;
+  // J (values only, no decisions): FMA forms.
+  const float du = (su0 - su1) * div;
+  const float dvv = (sv0 - sv1) * div;
+  const float iz = __builtin_amdgcn_rcpf(p.z);
+  const float fxz = L.fx * iz, fyz = L.fy * iz;
+  const float gx = du * fxz, gy = dvv * fyz;
+  const float gz = -fmaf(gx, p.x, gy * p.y) * iz;
+  j[0] = gx;
+  j[1] = gy;
+  j[2] = gz;
+  j[3] = fmaf(p.y, gz, -(p.z * gy));
+  j[4] = fmaf(p.z, gx, -(p.x * gz));
+  j[5] = fmaf(p.x, gy, -(p.y * gx));
+  rlast = res;
+  sq += res * res;
+  return sq;
+}
+
 // Point-to-plane and feature terms of source pixel idx when the map names it
 // as its target pixel's owner (src/icp_jacobian_geometric.cu:54-85,
 // src/icp_jacobian_feature.cu:58-109). The code has no branch: a pixel that
@@ -617,7 +804,8 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
 // arithmetic (the same decisions as the correspondence pass); the values of
 // J and r use FMA. BORN: this is the lane's first pixel, the slots start
 // from its terms.
-template <typename S, bool GEOM, bool FEAT, bool BORN, int NB = 4>
+template <typename S, bool GEOM, bool FEAT, bool BORN, int NB = 4,
+          bool IL = false>
 __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
                                          const LevelRes& R, const float* m,
                                          const unsigned long long* __restrict__ map,
@@ -667,7 +855,9 @@ __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
     // Lanes outside the target sample cell (0, 0): in range, unused.
     const float uu = inside ? u : 0.0f, vv = inside ? v : 0.0f;
     float jf[6], rl;
-    const float sq = feature_terms_res<S, NB>(L, idx, p, uu, vv, tw.fdiv, jf, rl);
+    const float sq =
+        IL ? feature_terms_il<S>(L, idx, p, uu, vv, tw.fdiv, jf, rl)
+           : feature_terms_res<S, NB>(L, idx, p, uu, vv, tw.fdiv, jf, rl);
     const bool use = own & !(sq > tw.fthr2);
     float j[6];
 #pragma unroll
@@ -684,7 +874,8 @@ __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
 }
 
 // jacobian_pixel_at with the source pixel's mask and point loaded here.
-template <typename S, bool GEOM, bool FEAT, bool BORN, int NB = 4>
+template <typename S, bool GEOM, bool FEAT, bool BORN, int NB = 4,
+          bool IL = false>
 __device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
                                       const float* m,
                                       const unsigned long long* __restrict__ map,
@@ -693,9 +884,9 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
                                       float (&acc)[kSlots]) {
   const uint8_t valid_src = res_u8(R.sm, int(idx));
   const V3<float> ps = res_ld3<S>(R.sp, int(idx));
-  jacobian_pixel_at<S, GEOM, FEAT, BORN, NB>(L, R, m, map, tag, idx,
-                                             in_run & (valid_src != 0), ps, tw,
-                                             acc);
+  jacobian_pixel_at<S, GEOM, FEAT, BORN, NB, IL>(L, R, m, map, tag, idx,
+                                                 in_run & (valid_src != 0), ps,
+                                                 tw, acc);
 }
 
 // G^T G of source pixel idx from its matched target point

@@ -12,6 +12,7 @@ import torch
 LIB_PATH = Path(__file__).resolve().parent / "libslamtb_hip.so"
 
 STB_F32, STB_F64, STB_BF16, STB_F16, STB_I32, STB_U8, STB_I64, STB_I16 = range(8)
+STB_FEAT_PLANAR, STB_FEAT_INTERLEAVED = 0, 1
 
 _TORCH_DTYPE = {
     torch.float32: STB_F32,
@@ -40,7 +41,7 @@ class StbIcpLevel(ctypes.Structure):
         ("tgt_mask", _vp), ("tgt_feats", _vp), ("n_src", _i64),
         ("height", _i32), ("width", _i32), ("channels", _i32),
         ("storage", _i32), ("fx", _f32), ("fy", _f32), ("cx", _f32),
-        ("cy", _f32),
+        ("cy", _f32), ("feat_layout", _i32), ("reserved", _i32),
     ]
 
 
@@ -71,7 +72,7 @@ class StbIcpBatchSizes(ctypes.Structure):
     _fields_ = [
         ("num_pairs", _i32), ("total_tiles", _i32), ("tile_px", _i32),
         ("storage", _i32), ("total_map", _i64), ("plan_bytes", _sz),
-        ("scratch_bytes", _sz), ("max_channels", _i32), ("reserved", _i32),
+        ("scratch_bytes", _sz), ("max_channels", _i32), ("feat_layout", _i32),
     ]
 
 

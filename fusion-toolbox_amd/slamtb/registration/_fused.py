@@ -21,13 +21,26 @@ _STORAGE = {torch.float32: _lib.STB_F32, torch.bfloat16: _lib.STB_BF16,
             torch.float16: _lib.STB_F16}
 
 
+FEAT_LAYOUTS = {"planar": _lib.STB_FEAT_PLANAR,
+                "interleaved": _lib.STB_FEAT_INTERLEAVED}
+
+
 class Level:
     """One pyramid level of one frame pair, as the C struct plus the tensors
-    it points to (kept alive here)."""
+    it points to (kept alive here).
+
+    ``feat_layout="interleaved"`` stores the features channel-innermost
+    (source [N, C], target [H, W, C]) instead of the reference's planar
+    [C, N] / [C, H, W]: the fused Jacobian then reads a texel's channels in
+    16-byte pieces. Inputs are given planar either way; the copy to the
+    interleaved layout is made here, once per level."""
 
     def __init__(self, kcam, src_points, src_normals, src_mask, tgt_points,
                  tgt_normals, tgt_mask, src_feats=None, tgt_feats=None,
-                 storage=None):
+                 storage=None, feat_layout="planar"):
+        if feat_layout not in FEAT_LAYOUTS:
+            raise ValueError(f"feat_layout must be one of {sorted(FEAT_LAYOUTS)}, "
+                             f"got {feat_layout!r}")
         _lib.require_gpu(src_points, src_normals, src_mask, tgt_points,
                          tgt_normals, tgt_mask, src_feats, tgt_feats,
                          what="ICPOdometry")
@@ -50,6 +63,7 @@ class Level:
         self.tgt_mask = tgt_mask.to(torch.bool).contiguous()
         self.channels = 0
         self.src_feats = self.tgt_feats = None
+        self.feat_layout = "planar"
         if src_feats is not None and tgt_feats is not None:
             C = tgt_feats.size(0)
             self.tgt_feats = tgt_feats.to(storage).contiguous()
@@ -57,6 +71,10 @@ class Level:
             if self.src_feats.size(1) != n:
                 raise RuntimeError("source features must be [C, N]")
             self.channels = C
+            if feat_layout == "interleaved":
+                self.tgt_feats = self.tgt_feats.permute(1, 2, 0).contiguous()
+                self.src_feats = self.src_feats.t().contiguous()
+                self.feat_layout = "interleaved"
         k = kcam.matrix if hasattr(kcam, "matrix") else kcam
         k = torch.as_tensor(k).detach().to("cpu", torch.float32)
         self.fx, self.fy = float(k[0, 0]), float(k[1, 1])
@@ -70,7 +88,7 @@ class Level:
             _lib.ptr(self.tgt_points), _lib.ptr(self.tgt_normals),
             _lib.ptr(self.tgt_mask), _lib.ptr(self.tgt_feats), n, H, W,
             self.channels, _STORAGE[storage], self.fx, self.fy, self.cx,
-            self.cy)
+            self.cy, FEAT_LAYOUTS[self.feat_layout], 0)
 
     @property
     def has_features(self):

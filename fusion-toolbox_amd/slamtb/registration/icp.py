@@ -132,13 +132,14 @@ class ICPOdometry:
 
     def level(self, kcam, source_points, source_normals, source_mask,
               source_feats, target_points, target_mask, target_normals,
-              target_feats, storage=None):
+              target_feats, storage=None, feat_layout="planar"):
         has_features = not None in (source_feats, target_feats,
                                     self._feature_step)
         return Level(kcam, source_points, source_normals, source_mask,
                      target_points, target_normals, target_mask,
                      source_feats if has_features else None,
-                     target_feats if has_features else None, storage=storage)
+                     target_feats if has_features else None, storage=storage,
+                     feat_layout=feat_layout)
 
     def enqueue(self, runner, level, transform=None, chain=False,
                 information=True):

@@ -35,8 +35,9 @@ class MultiscaleRegistration:
         return source_pyr, src_fp, target_pyr, tgt_fp
 
     def levels(self, source_frame, target_frame, source_feats=None,
-               target_feats=None, storage=None):
-        """Per-level fused inputs, coarse first (estimators must be ICP)."""
+               target_feats=None, storage=None, feat_layout="planar"):
+        """Per-level fused inputs, coarse first (estimators must be ICP);
+        storage and feat_layout as in ``_fused.Level``."""
         src_pyr, src_fp, tgt_pyr, tgt_fp = self.pyramids(
             source_frame, target_frame, source_feats, target_feats)
         out = []
@@ -44,7 +45,8 @@ class MultiscaleRegistration:
                                               src_fp, tgt_pyr, tgt_fp):
             out.append((est, est.level(src.kcam, src.points, src.normals,
                                        src.mask, sf, tgt.points, tgt.mask,
-                                       tgt.normals, tf, storage=storage)))
+                                       tgt.normals, tf, storage=storage,
+                                       feat_layout=feat_layout)))
         return out
 
     @staticmethod

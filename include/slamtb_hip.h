@@ -342,7 +342,20 @@ int stb_rigid_transform_backward(int32_t dtype, const void* matrix,
  * STB_F16) applies to points, normals and features; masks are bytes.
  * Source arrays are [n_src, 3] / [channels, n_src]; target arrays are
  * [height, width, 3] / [channels, height, width]. Set src_feats/tgt_feats to
- * NULL (or channels to 0) for a geometric-only level. */
+ * NULL (or channels to 0) for a geometric-only level.
+ * feat_layout STB_FEAT_INTERLEAVED stores the features channel-innermost
+ * instead: source [n_src, channels], target [height, width, channels]
+ * (the reference's feature maps are channel-planar; this layout has no
+ * counterpart there). A texel's channels are then read 16 bytes at a time:
+ * with BF16 coordinates, whose 8-bit mantissa scatters the projections of
+ * neighbouring pixels, that touches far fewer cache lines per pixel than
+ * one line per channel and tap. Results are bit-identical to the planar
+ * layout. It needs 16-bit storage and an even channel count of at most 32,
+ * and only the fused Gauss-Newton entry points (single-pair and batched)
+ * accept it. */
+#define STB_FEAT_PLANAR 0
+#define STB_FEAT_INTERLEAVED 1
+
 typedef struct stb_icp_level {
   const void* src_points;
   const void* src_normals;
@@ -358,6 +371,8 @@ typedef struct stb_icp_level {
   int32_t channels;
   int32_t storage;
   float fx, fy, cx, cy;
+  int32_t feat_layout;  /* STB_FEAT_PLANAR (0) or STB_FEAT_INTERLEAVED */
+  int32_t reserved;     /* 0 */
 } stb_icp_level;
 
 /* Options of ICPOdometry (slamtb/registration/icp.py:137-150). */
@@ -499,7 +514,7 @@ typedef struct stb_icp_batch_sizes {
   size_t plan_bytes;    /* size of the plan image */
   size_t scratch_bytes; /* device scratch: two maps per pair, tile partials */
   int32_t max_channels; /* most feature channels of any pair */
-  int32_t reserved;
+  int32_t feat_layout;  /* shared by every pair (stb_icp_level.feat_layout) */
 } stb_icp_batch_sizes;
 
 /* Host only. Builds the plan image of one pyramid level for `num_pairs`

@@ -127,3 +127,49 @@ def test_knn_size_limits_are_errors():
     rc = lib.stb_knn_grid(0, fake, i64, 2, 2, 2, 0.0, 0.0, 0.0, 0.1, 1 << 31,
                           fake, 10, None, 8, 0.1, fake, i64, None)
     assert rc == 1 and b"31-bit" in lib.stb_last_error()
+
+
+def test_level_struct_layout_and_interleaved_validation():
+    """The level descriptor ends with feat_layout (8-byte aligned, 112
+    bytes). Interleaved features are refused on the host, before anything
+    reaches the device, unless the storage is 16-bit and the channel count
+    even and at most 32: by the batch planner, and by the single-pair entry
+    points."""
+    lib = _lib.lib()
+    assert ctypes.sizeof(_lib.StbIcpLevel) == 112
+    assert _lib.StbIcpLevel.feat_layout.offset == 104
+
+    fake = ctypes.c_void_p(0x1000)
+
+    def level(storage, channels, layout):
+        return _lib.StbIcpLevel(fake, fake, fake, fake, fake, fake, fake, fake,
+                                64 * 48, 48, 64, channels, storage, 1.0, 1.0,
+                                0.0, 0.0, layout, 0)
+
+    sizes = _lib.StbIcpBatchSizes()
+    for storage, channels, ok in [(_lib.STB_BF16, 32, True),
+                                  (_lib.STB_F16, 6, True),
+                                  (_lib.STB_BF16, 3, False),
+                                  (_lib.STB_BF16, 34, False),
+                                  (_lib.STB_F32, 8, False)]:
+        arr = (_lib.StbIcpLevel * 2)(level(storage, channels, _lib.STB_FEAT_INTERLEAVED),
+                                     level(storage, channels, _lib.STB_FEAT_INTERLEAVED))
+        rc = lib.stb_icp_batch_plan(2, arr, 1024, None, 0, ctypes.byref(sizes))
+        assert (rc == 0) == ok, (storage, channels)
+        if ok:
+            assert sizes.feat_layout == _lib.STB_FEAT_INTERLEAVED
+    mixed = (_lib.StbIcpLevel * 2)(level(_lib.STB_BF16, 8, _lib.STB_FEAT_INTERLEAVED),
+                                   level(_lib.STB_BF16, 8, _lib.STB_FEAT_PLANAR))
+    assert lib.stb_icp_batch_plan(2, mixed, 1024, None, 0, ctypes.byref(sizes)) != 0
+    assert b"feature layout" in lib.stb_last_error()
+
+    params = _lib.StbIcpParams(10.0, 1.0, 0.1, 0.3, 2.75, 1, 1, 0, 0, 0)
+    for storage, channels in [(_lib.STB_F32, 8), (_lib.STB_BF16, 7)]:
+        lv = level(storage, channels, _lib.STB_FEAT_INTERLEAVED)
+        rc = lib.stb_icp_iterate(ctypes.byref(lv), ctypes.byref(params), fake,
+                                 1 << 30, 1, None)
+        assert rc != 0 and b"interleaved" in lib.stb_last_error()
+    lv = level(_lib.STB_BF16, 8, 7)
+    rc = lib.stb_icp_iterate(ctypes.byref(lv), ctypes.byref(params), fake,
+                             1 << 30, 1, None)
+    assert rc != 0 and b"feat_layout" in lib.stb_last_error()
