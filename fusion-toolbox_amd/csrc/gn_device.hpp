@@ -854,10 +854,16 @@ __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
     constexpr bool FB = BORN && !GEOM;
     // Lanes outside the target sample cell (0, 0): in range, unused.
     const float uu = inside ? u : 0.0f, vv = inside ? v : 0.0f;
-    float jf[6], rl;
-    const float sq =
-        IL ? feature_terms_il<S>(L, idx, p, uu, vv, tw.fdiv, jf, rl)
-           : feature_terms_res<S, NB>(L, idx, p, uu, vv, tw.fdiv, jf, rl);
+    float jf[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, rl = 0.0f, sq = 0.0f;
+    if (IL) {
+      // Interleaved (16-bit) features load for owners only: BF16
+      // coordinates collapse runs of source pixels onto one target pixel
+      // (about half of them own theirs at 1280x720), so skipping the
+      // others' 32-channel taps outweighs waiting for the key first.
+      if (own) sq = feature_terms_il<S>(L, idx, p, uu, vv, tw.fdiv, jf, rl);
+    } else {
+      sq = feature_terms_res<S, NB>(L, idx, p, uu, vv, tw.fdiv, jf, rl);
+    }
     const bool use = own & !(sq > tw.fthr2);
     float j[6];
 #pragma unroll
