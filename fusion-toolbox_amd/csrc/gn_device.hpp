@@ -507,6 +507,30 @@ __device__ inline float bil_mix_res(int W, const Bil<float>& b, const float t[4]
   return a * (1.0f - b.vr) + c * b.vr;
 }
 
+// The feature term's J from the derivatives du, dv of the last channel at
+// the projection of p (src/icp_jacobian_feature.cu:81-106,
+// include/slamtb/camera.hpp:65-78): feature_terms<..., FAST = true>'s
+// arithmetic, one hardware reciprocal of z (about 1 ulp) for the four
+// divides, no contraction.
+__device__ inline void feature_j(float fx, float fy, const V3<float>& p,
+                                 float du, float dv, float j[6]) {
+#pragma clang fp contract(off)
+  const float iz = __builtin_amdgcn_rcpf(p.z);
+  const float iz2 = iz * iz;
+  const float j00 = fx * iz;
+  const float j02 = -p.x * fx * iz2;
+  const float j11 = fy * iz;
+  const float j12 = -p.y * fy * iz2;
+  const V3<float> g{du * j00, dv * j11, du * j02 + dv * j12};
+  const V3<float> c = cross3(p, g);
+  j[0] = g.x;
+  j[1] = g.y;
+  j[2] = g.z;
+  j[3] = c.x;
+  j[4] = c.y;
+  j[5] = c.z;
+}
+
 // feature_terms<float, S, S, true, NB> (icp_math.hpp) with its taps and
 // source values read through buffer resources, the same samples in the same
 // arithmetic. (The finite differences subtract samples 0.1 px apart, so a
@@ -587,20 +611,7 @@ __device__ inline float feature_terms_res(const LevelView<S>& L, int64_t idx,
     }
   }
   const float res = last - flast;
-  // J (values only, no decisions): FMA forms.
-  const float du = (su0 - su1) * div;
-  const float dv = (sv0 - sv1) * div;
-  const float iz = __builtin_amdgcn_rcpf(p.z);
-  const float fxz = L.fx * iz, fyz = L.fy * iz;
-  const float gx = du * fxz, gy = dv * fyz;
-  // J of the projection: g = (du fx/z, dv fy/z, -(du fx x + dv fy y)/z^2).
-  const float gz = -fmaf(gx, p.x, gy * p.y) * iz;
-  j[0] = gx;
-  j[1] = gy;
-  j[2] = gz;
-  j[3] = fmaf(p.y, gz, -(p.z * gy));
-  j[4] = fmaf(p.z, gx, -(p.x * gz));
-  j[5] = fmaf(p.x, gy, -(p.y * gx));
+  feature_j(L.fx, L.fy, p, (su0 - su1) * div, (sv0 - sv1) * div, j);
   rlast = res;
   sq += res * res;
   return sq;
@@ -773,19 +784,7 @@ __device__ inline float feature_terms_il(const LevelView<S>& L, int64_t idx,
   const float su0 = bil_mix_res(W, bu0, t1), su1 = bil_mix_res(W, bu1, t2);
   const float sv0 = bil_mix_res(W, bv0, t3), sv1 = bil_mix_res(W, bv1, t4);
   const float res = last - flast;
-  // J (values only, no decisions): FMA forms.
-  const float du = (su0 - su1) * div;
-  const float dvv = (sv0 - sv1) * div;
-  const float iz = __builtin_amdgcn_rcpf(p.z);
-  const float fxz = L.fx * iz, fyz = L.fy * iz;
-  const float gx = du * fxz, gy = dvv * fyz;
-  const float gz = -fmaf(gx, p.x, gy * p.y) * iz;
-  j[0] = gx;
-  j[1] = gy;
-  j[2] = gz;
-  j[3] = fmaf(p.y, gz, -(p.z * gy));
-  j[4] = fmaf(p.z, gx, -(p.x * gz));
-  j[5] = fmaf(p.x, gy, -(p.y * gx));
+  feature_j(L.fx, L.fy, p, (su0 - su1) * div, (sv0 - sv1) * div, j);
   rlast = res;
   sq += res * res;
   return sq;
@@ -800,10 +799,10 @@ __device__ inline float feature_terms_il(const LevelView<S>& L, int64_t idx,
 // cannot leak into the sums). The owner word, the target point and normal
 // and the feature taps are loaded whatever the owner test says, so all of
 // them issue in one batch: a pixel costs two memory round trips. The
-// projection, the rounding and the owner test keep the reference's
-// arithmetic (the same decisions as the correspondence pass); the values of
-// J and r use FMA. BORN: this is the lane's first pixel, the slots start
-// from its terms.
+// projection, the rounding, the owner test and the terms keep the
+// reference's arithmetic (the same decisions as the correspondence pass,
+// the oracle's per-pixel products). BORN: this is the lane's first pixel,
+// the slots start from its terms.
 template <typename S, bool GEOM, bool FEAT, bool BORN, int NB = 4,
           bool IL = false>
 __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
@@ -836,15 +835,14 @@ __device__ inline void jacobian_pixel_at(const LevelView<S>& L,
   if (GEOM) {
     const V3<float> q = res_ld3<S>(R.tp, pos);
     const V3<float> qn = res_ld3<S>(R.tn, pos);
-    const V3<float> d = sub3(q, p);
-    const float r = fmaf(d.x, qn.x, fmaf(d.y, qn.y, d.z * qn.z));
+    // The reference's arithmetic (no contraction): the per-pixel products
+    // equal the oracle's, so sums differ from it in summation order only.
+    const float r = dot3(sub3(q, p), qn);
+    const V3<float> c = cross3(p, qn);
     const bool jon = own & (tw.gon != 0);
-    const float j[6] = {jon ? qn.x : 0.0f,
-                        jon ? qn.y : 0.0f,
-                        jon ? qn.z : 0.0f,
-                        jon ? fmaf(p.y, qn.z, -(p.z * qn.y)) : 0.0f,
-                        jon ? fmaf(p.z, qn.x, -(p.x * qn.z)) : 0.0f,
-                        jon ? fmaf(p.x, qn.y, -(p.y * qn.x)) : 0.0f};
+    const float j[6] = {jon ? qn.x : 0.0f, jon ? qn.y : 0.0f,
+                        jon ? qn.z : 0.0f, jon ? c.x : 0.0f,
+                        jon ? c.y : 0.0f,  jon ? c.z : 0.0f};
     const float rs = own ? r : 0.0f;
     acc_term<BORN>(acc, j, rs);
     acc[27] = BORN ? rs * rs : fmaf(rs, rs, acc[27]);
