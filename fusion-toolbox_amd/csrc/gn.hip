@@ -252,6 +252,92 @@ __global__ void __launch_bounds__(BLOCK)
   if (rec && blockIdx.x == 0) commit_record(st, rec, tw.gw, tw.fw);
 }
 
+// jacobian_fused_kernel as a software pipeline over up to three pixels per
+// thread (FP32 storage, planar features whose channels other than the last
+// fit one batch of NB, or none). All three pixels' source data load first;
+// the second pixel's gathers issue before the first pixel's terms, and in
+// wavefronts with a third pixel its gathers issue before the second's
+// terms, so that a pixel's gathers are in flight while the VALU works on
+// the one before. Blocks of 512 threads leave two wavefronts per SIMD, each
+// with room in its registers for two pixels' gathered data. The block
+// partials keep their meaning (each block sums its run of pixels); only the
+// order of the FP32 lane sums differs from jacobian_fused_kernel's.
+// Requires chunk <= 3 * BLOCK (pipe_fits checks it on the host).
+template <bool GEOM, bool FEAT, int NB, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+    jacobian_pipe_kernel(stb_icp_level Ld, stb_icp_state* __restrict__ st,
+                         const SolveRecord* __restrict__ rec,
+                         unsigned long long* __restrict__ map0,
+                         unsigned long long* __restrict__ map1,
+                         int64_t map_len, double* __restrict__ partials,
+                         TermWeights tw, int fixed_map, int64_t chunk) {
+  __shared__ double sh[(BLOCK / 64) * kSlots];
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const LevelView<float> L = view_of<float>(Ld);
+  const LevelRes R = res_of(L);
+  const int64_t lo = int64_t(blk) * chunk;
+  const int64_t hi = lo + chunk < L.n ? lo + chunk : L.n;
+  const int64_t i0 = lo + threadIdx.x, i1 = i0 + BLOCK, i2 = i1 + BLOCK;
+  // A thread without a second or third pixel re-reads its first (a cache
+  // hit) and evaluates it as invalid.
+  const int64_t j1 = i1 < hi ? i1 : i0, j2 = i2 < hi ? i2 : i0;
+  const uint8_t vs0 = res_u8(R.sm, int(i0));
+  const V3<float> ps0 = res_ld3<float>(R.sp, int(i0));
+  const uint8_t vs1 = res_u8(R.sm, int(j1));
+  const V3<float> ps1 = res_ld3<float>(R.sp, int(j1));
+  const uint8_t vs2 = res_u8(R.sm, int(j2));
+  const V3<float> ps2 = res_ld3<float>(R.sp, int(j2));
+  __builtin_amdgcn_sched_barrier(0);
+  const int32_t* __restrict__ selp = rec ? &rec->map_sel : &st->map_sel;
+  const int32_t* __restrict__ tagp = rec ? &rec->map_epoch : &st->map_epoch;
+  const float* __restrict__ rt = rec ? rec->rt : st->rt;
+  const int sel = *selp;
+  const uint32_t tag = fixed_map ? kFreshTag : uint32_t(*tagp);
+  const unsigned long long* __restrict__ map = (sel ^ fixed_map) ? map1 : map0;
+  float m[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) m[i] = rt[i];
+  const __amdgpu_buffer_rsrc_t rmap = make_res(map, L.HW);
+  PxGather<NB> X0, X1;
+  // Scheduling barriers keep each pixel's loads together and in order: the
+  // waits of a pixel's terms then leave the later pixels' loads in flight.
+  jac_issue<GEOM, FEAT, NB>(L, R, m, rmap, i0, (i0 < hi) & (vs0 != 0), ps0, X0);
+  __builtin_amdgcn_sched_barrier(0);
+  jac_issue<GEOM, FEAT, NB>(L, R, m, rmap, j1, (i1 < hi) & (vs1 != 0), ps1, X1);
+  __builtin_amdgcn_sched_barrier(0);
+  float acc[kSlots];
+  jac_consume<GEOM, FEAT, true, NB>(L, X0, uint32_t(i0), tag, tw, acc);
+  uint32_t k1 = uint32_t(j1);
+  settle(acc, X1, k1);
+  __builtin_amdgcn_sched_barrier(0);
+  // Whether any lane of this wavefront has a third pixel (uniform). Each
+  // branch is straight-line, so its waits count only its own loads.
+  const int64_t wave0 =
+      int64_t(__builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u);
+  if (lo + wave0 + 2 * BLOCK < hi) {
+    PxGather<NB> X2;
+    jac_issue<GEOM, FEAT, NB>(L, R, m, rmap, j2, (i2 < hi) & (vs2 != 0), ps2,
+                              X2);
+    __builtin_amdgcn_sched_barrier(0);
+    jac_consume<GEOM, FEAT, false, NB>(L, X1, k1, tag, tw, acc);
+    uint32_t k2 = uint32_t(j2);
+    settle(acc, X2, k2);
+    __builtin_amdgcn_sched_barrier(0);
+    jac_consume<GEOM, FEAT, false, NB>(L, X2, k2, tag, tw, acc);
+  } else {
+    jac_consume<GEOM, FEAT, false, NB>(L, X1, k1, tag, tw, acc);
+  }
+  if (!fixed_map && tag == 0) {
+    unsigned long long* __restrict__ next = sel ? map0 : map1;
+    const int64_t mchunk = (map_len + gridDim.x - 1) / gridDim.x;
+    const int64_t mlo = int64_t(blk) * mchunk;
+    const int64_t mhi = mlo + mchunk < map_len ? mlo + mchunk : map_len;
+    for (int64_t i = mlo + threadIdx.x; i < mhi; i += BLOCK) next[i] = kKeySentinel;
+  }
+  block_partial<BLOCK>(acc, partials + int64_t(blk) * kSlots, sh, tw.b);
+  if (rec && blockIdx.x == 0) commit_record(st, rec, tw.gw, tw.fw);
+}
+
 // The fixed-order sum of `count` partials by one block -- the order every
 // correspondence launch that applies an update uses too, so a loop and the
 // split entry points give the same bits -- then the pose update, the stored
@@ -461,6 +547,32 @@ void launch_jacobian_nb(const stb_icp_level& L, const JacShape& js,
                        fixed_map, chunk);
 }
 
+// The pipelined Jacobian kernel's block, and whether it takes a launch:
+// FP32 storage, the wide-level shape (kSolveBlock blocks looping over runs of
+// pixels) with runs of at most three pixels per thread, and planar features
+// whose channels other than the last fit one batch.
+constexpr int kPipeBlock = 512;
+
+bool pipe_fits(const stb_icp_level& L, const JacShape& js, bool feat, int nb) {
+  if (L.storage != STB_F32 || js.block != kSolveBlock || js.one_px) return false;
+  if (feat && (L.feat_layout != STB_FEAT_PLANAR || L.channels - 1 > nb))
+    return false;
+  const int64_t chunk = (L.n_src + js.grid - 1) / js.grid;
+  return chunk <= 3 * kPipeBlock;
+}
+
+template <bool GEOM, bool FEAT, int NB>
+void launch_jacobian_pipe(const stb_icp_level& L, const JacShape& js,
+                          const Workspace& w, const SolveRecord* rec,
+                          const TermWeights& tw, int fixed_map,
+                          hipStream_t stream) {
+  const int64_t chunk = (L.n_src + js.grid - 1) / js.grid;
+  hipLaunchKernelGGL((jacobian_pipe_kernel<GEOM, FEAT, NB, kPipeBlock>),
+                     dim3(js.grid), dim3(kPipeBlock), 0, stream, L, w.st, rec,
+                     w.map[0], w.map[1], w.map_len, w.partials, tw, fixed_map,
+                     chunk);
+}
+
 // The Jacobian launch, then (do_solve >= 0) the launch that sums its
 // partials and applies them: a pose update (1) or the stored linearisation
 // (0). do_solve < 0: the Jacobian launch alone.
@@ -473,7 +585,13 @@ void launch_jacobian(const stb_icp_level& L, const stb_icp_params& P,
   // Feature channels other than the last load in batches of four, or of two
   // when there are fewer than four of them (three-channel colour: no
   // padding slots); interleaved features in chunks of eight.
-  if (FEAT && L.feat_layout == STB_FEAT_INTERLEAVED)
+  const int nb = L.channels - 1 < 4 ? 2 : 4;
+  if (std::is_same<S, float>::value && pipe_fits(L, js, FEAT, nb)) {
+    if (!FEAT || nb == 2)
+      launch_jacobian_pipe<GEOM, FEAT, 2>(L, js, w, rec, tw, fixed_map, stream);
+    else
+      launch_jacobian_pipe<GEOM, FEAT, 4>(L, js, w, rec, tw, fixed_map, stream);
+  } else if (FEAT && L.feat_layout == STB_FEAT_INTERLEAVED)
     launch_jacobian_nb<S, GEOM, FEAT, 4, true>(L, js, w, rec, tw, fixed_map,
                                                stream);
   else if (FEAT && L.channels - 1 < 4)

@@ -433,6 +433,9 @@ __device__ inline void res_taps_f32(__amdgpu_buffer_rsrc_t r, int W,
   res_ld_f2(r, o + W * 4, t[2], t[3]);
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
 // The taps of the last channel's five samples (c0 at (u, v), bu0/bu1 at
 // u +- h, bv0/bv1 at v +- h) from four loads instead of ten. The u samples'
 // cells lie one column left of, at, or one column right of c0's, so two
@@ -442,28 +445,41 @@ __device__ inline void res_taps_f32(__amdgpu_buffer_rsrc_t r, int W,
 // fu + 1 add two 8-byte loads (read only when a sample's cell moved there;
 // past the channel's start or end they read 0). Each sample then selects
 // its taps from registers.
-__device__ inline void res_taps5_f32(__amdgpu_buffer_rsrc_t r, int W,
-                                     const Bil<float>& c0, const Bil<float>& bu0,
-                                     const Bil<float>& bu1, const Bil<float>& bv0,
-                                     const Bil<float>& bv1, float t0[4],
-                                     float t1[4], float t2[4], float t3[4],
-                                     float t4[4]) {
+// The raw words of the last channel's four row windows (res_taps5_f32).
+struct Taps5Raw {
+  u32x4 a, b;   // rows fv, fv + 1 from column fu - 1 (0 when fu = 0)
+  u32x2 up, dn; // rows fv - 1, fv + 2 at columns fu, fu + 1
+};
+
+__device__ inline Taps5Raw taps5_load(__amdgpu_buffer_rsrc_t r, int W,
+                                      const Bil<float>& c0) {
   const int o = mad24(c0.vi, W, c0.ui) * 4;  // byte offset of (fu, fv)
+  const int ow = c0.ui >= 1 ? o - 4 : o;     // the 4-texel window's first column
+  Taps5Raw w;
+  w.a = __builtin_amdgcn_raw_buffer_load_b128(r, ow, 0, 0);
+  w.b = __builtin_amdgcn_raw_buffer_load_b128(r, ow + W * 4, 0, 0);
+  w.up = __builtin_amdgcn_raw_buffer_load_b64(r, o - W * 4, 0, 0);
+  w.dn = __builtin_amdgcn_raw_buffer_load_b64(r, o + W * 8, 0, 0);
+  return w;
+}
+
+// Each of the five samples' taps, selected from the windows.
+__device__ inline void taps5_select(const Taps5Raw& w, const Bil<float>& c0,
+                                    const Bil<float>& bu0, const Bil<float>& bu1,
+                                    const Bil<float>& bv0, const Bil<float>& bv1,
+                                    float t0[4], float t1[4], float t2[4],
+                                    float t3[4], float t4[4]) {
   const bool left = c0.ui >= 1;
-  const int ow = left ? o - 4 : o;  // the 4-texel window's first column
-  const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, ow, 0, 0);
-  const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, ow + W * 4, 0, 0);
-  float up0, up1, dn0, dn1;
-  res_ld_f2(r, o - W * 4, up0, up1);
-  res_ld_f2(r, o + W * 8, dn0, dn1);
+  const float up0 = __uint_as_float(w.up[0]), up1 = __uint_as_float(w.up[1]);
+  const float dn0 = __uint_as_float(w.dn[0]), dn1 = __uint_as_float(w.dn[1]);
   // Rows fv (A) and fv + 1 (B) at columns fu - 1 .. fu + 2.
   float A[4], B[4];
-  A[0] = __uint_as_float(a[0]);
-  B[0] = __uint_as_float(b[0]);
+  A[0] = __uint_as_float(w.a[0]);
+  B[0] = __uint_as_float(w.b[0]);
 #pragma unroll
   for (int k = 1; k < 4; ++k) {
-    A[k] = __uint_as_float(left ? a[k] : a[k - 1]);
-    B[k] = __uint_as_float(left ? b[k] : b[k - 1]);
+    A[k] = __uint_as_float(left ? w.a[k] : w.a[k - 1]);
+    B[k] = __uint_as_float(left ? w.b[k] : w.b[k - 1]);
   }
   t0[0] = A[1];
   t0[1] = A[2];
@@ -489,6 +505,15 @@ __device__ inline void res_taps5_f32(__amdgpu_buffer_rsrc_t r, int W,
   t4[1] = uv ? up1 : A[2];
   t4[2] = uv ? A[1] : B[1];
   t4[3] = uv ? A[2] : B[2];
+}
+
+__device__ inline void res_taps5_f32(__amdgpu_buffer_rsrc_t r, int W,
+                                     const Bil<float>& c0, const Bil<float>& bu0,
+                                     const Bil<float>& bu1, const Bil<float>& bv0,
+                                     const Bil<float>& bv1, float t0[4],
+                                     float t1[4], float t2[4], float t3[4],
+                                     float t4[4]) {
+  taps5_select(taps5_load(r, W, c0), c0, bu0, bu1, bv0, bv1, t0, t1, t2, t3, t4);
 }
 
 // bil_mix for taps read through a resource over one channel: the taps of
@@ -891,6 +916,167 @@ __device__ inline void jacobian_pixel(const LevelView<S>& L, const LevelRes& R,
   jacobian_pixel_at<S, GEOM, FEAT, BORN, NB, IL>(L, R, m, map, tag, idx,
                                                  in_run & (valid_src != 0), ps,
                                                  tw, acc);
+}
+
+// ---------------------------------------------------------------------------
+// The Jacobian pixel in two parts, for a software pipeline over a thread's
+// pixels (FP32 storage, planar features whose channels other than the last
+// fit one batch of NB, or no features)
+// ---------------------------------------------------------------------------
+//
+// A launch whose wavefronts all start together runs every pixel's chain --
+// source loads, projection, gathers, terms -- in step across the chip, so
+// the terms' VALU work and the gathers' memory time follow each other
+// instead of overlapping. jac_issue does everything up to and including the
+// issue of a pixel's gathers and consumes none of their results; jac_consume
+// does the rest. Issuing the next pixel's gathers before consuming the
+// current pixel's keeps them in flight while the VALU works. The arithmetic
+// is jacobian_pixel_at's, the same decisions and per-pixel products.
+template <int NB>
+struct PxGather {
+  V3<float> p;   // transformed source point
+  float u, v;    // its projection, (0, 0) outside the target
+  bool inside;   // valid source pixel projecting into the target
+  uint32_t klo, khi;  // the map key at the target pixel
+  V3<float> q, qn;    // target point and normal
+  Taps5Raw last;      // the last channel's windows
+  u32x2 tp[NB][2];    // the other channels' two tap rows
+  float fs[NB], flast;  // source feature values
+};
+
+template <bool GEOM, bool FEAT, int NB>
+__device__ inline void jac_issue(const LevelView<float>& L, const LevelRes& R,
+                                 const float* m, __amdgpu_buffer_rsrc_t rmap,
+                                 int64_t idx, bool valid_src, V3<float> ps,
+                                 PxGather<NB>& X) {
+  float u, v;
+  {
+#pragma clang fp contract(off)
+    X.p = rigid_point(m, ps);
+    project(L.fx, L.fy, L.cx, L.cy, X.p, u, v);
+  }
+  int ui, vi;
+  const bool in_u = round_px(u, L.W, ui);
+  const bool in_v = round_px(v, L.H, vi);
+  X.inside = valid_src && in_u && in_v;
+  X.u = X.inside ? u : 0.0f;
+  X.v = X.inside ? v : 0.0f;
+  const int pos = X.inside ? mad24(vi, L.W, ui) : 0;
+  X.klo = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos, 0, 0);
+  X.khi = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos + 4, 0, 0);
+  if (GEOM) {
+    X.q = res_ld3<float>(R.tp, pos);
+    X.qn = res_ld3<float>(R.tn, pos);
+  }
+  if (FEAT) {
+    const int C = L.C;
+    const Bil<float> c0 = make_bil(X.u, X.v);
+    X.last = taps5_load(make_res(L.tf + int64_t(C - 1) * L.HW, L.HW), L.W, c0);
+    const __amdgpu_buffer_rsrc_t rs = make_res(L.sf, int64_t(C) * L.n);
+    const int sch = int(L.n) * 4;
+    const int so = int(idx) * 4;
+    X.flast = res_ld<float>(rs, so + (C - 1) * sch, 0);
+    const int last_c = C - 2 > 0 ? C - 2 : 0;
+    const int o = mad24(c0.vi, L.W, c0.ui) * 4;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const int c = k < last_c ? k : last_c;
+      const __amdgpu_buffer_rsrc_t rc = make_res(L.tf + int64_t(c) * L.HW, L.HW);
+      X.tp[k][0] = __builtin_amdgcn_raw_buffer_load_b64(rc, o, 0, 0);
+      X.tp[k][1] = __builtin_amdgcn_raw_buffer_load_b64(rc, o + L.W * 4, 0, 0);
+      X.fs[k] = res_ld<float>(rs, so + c * sch, 0);
+    }
+  }
+}
+
+// The ordering point between two pixels of the pipeline. Scheduling
+// barriers order the loads, but the instruction selector places pure
+// arithmetic anywhere its operands allow, and it pulled the next pixel's
+// first uses of its gathers (and with them the waits for its loads) above
+// the current pixel's terms. Here every slot sum and every input of the
+// next pixel's terms that is not a gathered word (its point, projection
+// and index) pass through empty volatile asm statements, which keep their
+// order: the next pixel's terms, and its waits, come after the current
+// pixel's terms are in the slots. (Passing the gathered words themselves
+// through such a statement made the register allocator copy them, and a
+// copy waits for its load.)
+template <int NB>
+__device__ inline void settle(float (&a)[kSlots], PxGather<NB>& X,
+                              uint32_t& idx) {
+  asm volatile("" : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(a[4]),
+               "+v"(a[5]), "+v"(a[6]), "+v"(a[7]), "+v"(a[8]), "+v"(a[9]),
+               "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]),
+               "+v"(a[15]));
+  asm volatile("" : "+v"(a[16]), "+v"(a[17]), "+v"(a[18]), "+v"(a[19]),
+               "+v"(a[20]), "+v"(a[21]), "+v"(a[22]), "+v"(a[23]), "+v"(a[24]),
+               "+v"(a[25]), "+v"(a[26]), "+v"(a[27]), "+v"(a[28]), "+v"(a[29]),
+               "+v"(a[30]), "+v"(a[31]));
+  asm volatile("" : "+v"(X.p.x), "+v"(X.p.y), "+v"(X.p.z), "+v"(X.u),
+               "+v"(X.v), "+v"(idx));
+}
+
+template <bool GEOM, bool FEAT, bool BORN, int NB>
+__device__ inline void jac_consume(const LevelView<float>& L,
+                                   const PxGather<NB>& X, uint32_t idx,
+                                   uint32_t tag, const TermWeights& tw,
+                                   float (&acc)[kSlots]) {
+  const bool own =
+      X.inside & key_owned((uint64_t(X.khi) << 32) | X.klo, idx, tag);
+  const V3<float> p = X.p;
+  if (GEOM) {
+    const float r = dot3(sub3(X.q, p), X.qn);
+    const V3<float> c = cross3(p, X.qn);
+    const bool jon = own & (tw.gon != 0);
+    const float j[6] = {jon ? X.qn.x : 0.0f, jon ? X.qn.y : 0.0f,
+                        jon ? X.qn.z : 0.0f, jon ? c.x : 0.0f,
+                        jon ? c.y : 0.0f,    jon ? c.z : 0.0f};
+    const float rs = own ? r : 0.0f;
+    acc_term<BORN>(acc, j, rs);
+    acc[27] = BORN ? rs * rs : fmaf(rs, rs, acc[27]);
+    acc[29] = acc_add<BORN>(acc[29], own ? 1.0f : 0.0f);
+  }
+  if (FEAT) {
+#pragma clang fp contract(off)
+    constexpr bool FB = BORN && !GEOM;
+    const int W = L.W, C = L.C;
+    const float h = kFeatStep, u = X.u, v = X.v;
+    const Bil<float> c0 = make_bil(u, v);
+    const Bil<float> bu0 = make_bil(u + h, v);
+    const Bil<float> bu1 = make_bil(fmax(u - h, 0.0f), v);
+    const Bil<float> bv0 = make_bil(u, v + h);
+    const Bil<float> bv1 = make_bil(u, fmax(v - h, 0.0f));
+    float t0[4], t1[4], t2[4], t3[4], t4[4];
+    taps5_select(X.last, c0, bu0, bu1, bv0, bv1, t0, t1, t2, t3, t4);
+    const float last = bil_mix_res(W, c0, t0);
+    const float su0 = bil_mix_res(W, bu0, t1), su1 = bil_mix_res(W, bu1, t2);
+    const float sv0 = bil_mix_res(W, bv0, t3), sv1 = bil_mix_res(W, bv1, t4);
+    float sq = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const float t[4] = {__uint_as_float(X.tp[k][0][0]),
+                          __uint_as_float(X.tp[k][0][1]),
+                          __uint_as_float(X.tp[k][1][0]),
+                          __uint_as_float(X.tp[k][1][1])};
+      const float res = bil_mix_res(W, c0, t) - X.fs[k];
+      sq += (k < C - 1) ? res * res : 0.0f;
+    }
+    const float rl = last - X.flast;
+    float jf[6];
+    feature_j(L.fx, L.fy, p, (su0 - su1) * tw.fdiv, (sv0 - sv1) * tw.fdiv, jf);
+    sq += rl * rl;
+    const bool use = own & !(sq > tw.fthr2);
+    float j[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) j[a] = use ? jf[a] : 0.0f;
+    acc_term<FB>(acc, j, use ? -rl : 0.0f);
+    acc[28] = acc_add<FB>(acc[28], use ? sq : 0.0f);
+    acc[30] = acc_add<FB>(acc[30], use ? 1.0f : 0.0f);
+  }
+  if (BORN) {
+    if (!GEOM) acc[27] = acc[29] = 0.0f;
+    if (!FEAT) acc[28] = acc[30] = 0.0f;
+    acc[31] = 0.0f;
+  }
 }
 
 // G^T G of source pixel idx from its matched target point

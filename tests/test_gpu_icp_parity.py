@@ -151,9 +151,14 @@ def rel(a, b):
                  max(np.linalg.norm(np.asarray(b)), 1e-300))
 
 
+# 320x240: 256-thread Jacobian blocks. Above 131,072 pixels the FP32 launch
+# pipelines up to three pixels per thread over 256 runs of pixels: 560x320
+# leaves threads without a second pixel, 640x480 gives some wavefronts a
+# third, 640x560 most of them.
+@pytest.mark.parametrize("size", [(320, 240), (560, 320), (640, 480), (640, 560)])
 @pytest.mark.parametrize("terms", ["geom", "feat", "both"])
-def test_fused_reduction_matches_oracle_sums(terms):
-    P = prepared_pair(9, 320, 240, channels=3)
+def test_fused_reduction_matches_oracle_sums(terms, size):
+    P = prepared_pair(9, size[0], size[1], channels=3)
     T = small_motion(3, 1.0, 0.01)
     gw = 1.0 if terms in ("geom", "both") else 0.0
     fw = 0.5 if terms in ("feat", "both") else 0.0
