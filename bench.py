@@ -63,7 +63,10 @@ sys.path.insert(0, str(ROOT / "fusion-toolbox_amd"))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "ICP GN iters/s @640x480; batched pairs/s vs plain; %HBM peak; pose err"
-TRAFFIC_FILE = ROOT / "profiles" / "jacobian_fused_kernel_640x480_traffic.json"
+# The FP32 Jacobian launch of the 640x480 level (pipelined: up to three
+# pixels per thread in blocks of 512).
+JAC_KERNEL = "jacobian_pipe_kernel"
+TRAFFIC_FILE = ROOT / "profiles" / "jacobian_pipe_kernel_640x480_traffic.json"
 KW = dict(geom_weight=10.0, feat_weight=1.0, distance_threshold=0.1,
           normals_angle_thresh=math.radians(20), feat_residual_thresh=2.75)
 
@@ -1107,11 +1110,11 @@ def main():
         dom = dict(kernel="corresp_fused_kernel", ms=corr_ms, bytes=bytes_a,
                    bytes_all=bytes_a_all)
     else:
-        dom = dict(kernel="jacobian_fused_kernel", ms=jac_ms, bytes=bytes_b,
+        dom = dict(kernel=JAC_KERNEL, ms=jac_ms, bytes=bytes_b,
                    bytes_all=bytes_b_all)
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     traffic = None
-    if TRAFFIC_FILE.exists() and dom["kernel"] == "jacobian_fused_kernel":
+    if TRAFFIC_FILE.exists() and dom["kernel"] == JAC_KERNEL:
         traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
     _, copy_gbs = copy_rate(256)
 
@@ -1151,7 +1154,7 @@ def main():
         # previous update (corresp_solve_kernel) and the Jacobian launch; the
         # plain correspondence kernel runs only in a loop's first iteration.
         "launch_ms": {"corresp_fused_kernel": corr_ms,
-                      "jacobian_fused_kernel": jac_ms,
+                      JAC_KERNEL: jac_ms,
                       "corresp_solve_kernel_and_gaps": statistics.median(reruns) - jac_ms},
     }
     c1_levels = None
