@@ -66,7 +66,7 @@ template <typename S, bool TWO_PX>
 __global__ void __launch_bounds__(kSolveBlock)
     corresp_solve_kernel(stb_icp_level Ld, const stb_icp_state* __restrict__ st,
                          SolveRecord* __restrict__ rec,
-                         const double* __restrict__ partials, int count,
+                         const float* __restrict__ partials, int count,
                          unsigned long long* __restrict__ map0,
                          unsigned long long* __restrict__ map1, float dthr2,
                          float cstar) {
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(BLOCK)
                           const SolveRecord* __restrict__ rec,
                           unsigned long long* __restrict__ map0,
                           unsigned long long* __restrict__ map1,
-                          int64_t map_len, double* __restrict__ partials,
+                          int64_t map_len, float* __restrict__ partials,
                           TermWeights tw, int fixed_map, int64_t chunk) {
   __shared__ double sh[(BLOCK / 64) * kSlots];
   // Block `blk` takes the blk-th run of pixels and leaves the blk-th partial
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(BLOCK)
                          const SolveRecord* __restrict__ rec,
                          unsigned long long* __restrict__ map0,
                          unsigned long long* __restrict__ map1,
-                         int64_t map_len, double* __restrict__ partials,
+                         int64_t map_len, float* __restrict__ partials,
                          TermWeights tw, int fixed_map, int64_t chunk) {
   __shared__ double sh[(BLOCK / 64) * kSlots];
   const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(BLOCK)
 // linearisation or the information matrix (`mode`, a FinishMode).
 __global__ void __launch_bounds__(kSolveBlock)
     solve_fused_kernel(stb_icp_state* __restrict__ st,
-                       const double* __restrict__ partials, int count,
+                       const float* __restrict__ partials, int count,
                        double gw, double fw, int mode, int fixed_map) {
   __shared__ double sh[kSolveBlock];
   __shared__ double tot[kSlots];
@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(kBlock)
     information_fused_kernel(stb_icp_level Ld, stb_icp_state* __restrict__ st,
                              unsigned long long* __restrict__ map0,
                              unsigned long long* __restrict__ map1,
-                             double* __restrict__ partials) {
+                             float* __restrict__ partials) {
   __shared__ double sh[kShared];
   const unsigned long long* __restrict__ map = pick_map(st, map0, map1, 1);
   float m[12];
@@ -424,7 +424,7 @@ constexpr int kCorrespSolveBlocks = 256;
 struct Workspace {
   stb_icp_state* st;
   unsigned long long* map[2];
-  double* partials;
+  float* partials;  // FP32 block partials (block_partial)
   SolveRecord* rec;
   int64_t map_len;
 };
@@ -446,7 +446,7 @@ Workspace carve(void* base, int32_t H, int32_t W) {
   p += align256(size_t(w.map_len) * 8);
   w.map[1] = reinterpret_cast<unsigned long long*>(p);
   p += align256(size_t(w.map_len) * 8);
-  w.partials = reinterpret_cast<double*>(p);
+  w.partials = reinterpret_cast<float*>(p);
   p += align256(size_t(kMaxUpdateBlocks) * kSlots * sizeof(double));
   w.rec = reinterpret_cast<SolveRecord*>(p);
   return w;

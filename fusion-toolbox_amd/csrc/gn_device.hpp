@@ -5,9 +5,10 @@
 //     descriptor (any map storage type, FP32 math);
 //   * the slot reduction: 21 unique JtJ + 6 Jr + r2/count bookkeeping per
 //     lane in FP32, recursive halving across the wavefront (permlane swaps
-//     and DPP), FP64 across waves through LDS, one FP64 partial per block,
-//     and a fixed-order FP64 sum of the partials in a following launch (by
-//     one block, or by every block of a loop's next correspondence launch);
+//     and DPP), FP64 across waves through LDS, one partial per block (FP32
+//     single-pair, FP64 batched), and a fixed-order FP64 sum of the partials
+//     in a following launch (by one block, or by every block of a loop's
+//     next correspondence launch);
 //   * the per-pair state update: FP64 Cholesky solve and exp_rt_to_matrix,
 //     in one thread or redundantly in every lane of one wavefront.
 #pragma once
@@ -1178,15 +1179,20 @@ __device__ inline void wave_reduce_slots(float (&v)[kSlots], int lane) {
   v[0] += dpp_read<kDppQuad1032>(v[0]);
 }
 
-// Reduces the block's slots and stores them in FP64 at `out` (32 doubles),
-// with no fence: every consumer runs in a later launch (a fence and a ticket
-// for a consumer in the same launch cost each block ~20 ns at the memory
-// side). The wavefront sums enter FP64 scaled by the weight the per-pixel
-// sums left out (TermWeights): b^2 for the 21 JtJ slots, b for the 6 Jr
-// slots, 1 for the bookkeeping (b = 1: the slots as accumulated). `sh` is
-// LDS of BLOCK / 64 * kSlots doubles (kShared for kBlock).
-template <int BLOCK = kBlock>
-__device__ inline void block_partial(float (&acc)[kSlots], double* out,
+// Reduces the block's slots in FP64 and stores them at `out` (32 values of
+// type P), with no fence: every consumer runs in a later launch (a fence and
+// a ticket for a consumer in the same launch cost each block ~20 ns at the
+// memory side). The wavefront sums enter FP64 scaled by the weight the
+// per-pixel sums left out (TermWeights): b^2 for the 21 JtJ slots, b for the
+// 6 Jr slots, 1 for the bookkeeping (b = 1: the slots as accumulated). `sh`
+// is LDS of BLOCK / 64 * kSlots doubles (kShared for kBlock). The
+// single-pair launches store FP32 partials (P = float): every block of the
+// loop's next correspondence launch reads all of them, and half the bytes
+// save ~0.45 us of a 14 us iteration at 640x480, while the rounding to FP32
+// (6e-8 relative per block) matches that of the FP32 wavefront sums the
+// partial is made of. The sums over partials stay FP64.
+template <int BLOCK = kBlock, typename P>
+__device__ inline void block_partial(float (&acc)[kSlots], P* out,
                                      double* sh, double b = 1.0) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -1199,12 +1205,12 @@ __device__ inline void block_partial(float (&acc)[kSlots], double* out,
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < BLOCK / 64; ++w) s += sh[w * kSlots + threadIdx.x];
-    out[threadIdx.x] = s;
+    out[threadIdx.x] = P(s);
   }
   __syncthreads();
 }
 
-// Fixed-order FP64 sum of `count` partials (32 doubles each) into tot[32] by
+// Fixed-order FP64 sum of `count` partials (32 values each) into tot[32] by
 // a block of BLOCK threads, the same order whichever block performs it:
 // thread group g of BLOCK / 32 takes partials g, g + BLOCK / 32, ... in
 // batches of kSumBatch summed pairwise; the two groups of a wavefront are
@@ -1219,8 +1225,8 @@ constexpr int kSumBatch = 8;
 // One batch of sum_partials' loads: partials b0, b0 + BLOCK / 32, ... of
 // this thread's slot. Loads past the end re-read the last partial and are
 // dropped, so the batch issues without branches.
-template <int BLOCK>
-__device__ inline void load_partial_batch(const double* partials, int count,
+template <int BLOCK, typename P>
+__device__ inline void load_partial_batch(const P* partials, int count,
                                           int b0, double (&v)[kSumBatch]) {
   constexpr int kG = BLOCK / kSlots;
   const int slot = threadIdx.x & (kSlots - 1);
@@ -1228,16 +1234,16 @@ __device__ inline void load_partial_batch(const double* partials, int count,
   for (int u = 0; u < kSumBatch; ++u) {
     const int b = b0 + u * kG;
     const double x = partials[int64_t(b < count ? b : count - 1) * kSlots + slot];
-    v[u] = b < count ? x : 0.0;
+    v[u] = b < count ? x : 0.0;  // (FP32 partials widen exactly)
   }
 }
 
 // sum_partials with the thread's first batch already loaded (by
 // load_partial_batch at b0 = threadIdx.x / 32), so a caller can issue other
 // loads behind it: loads return in issue order.
-template <int BLOCK>
+template <int BLOCK, typename P>
 __device__ inline void sum_partials_from(double (&v)[kSumBatch],
-                                         const double* partials, int count,
+                                         const P* partials, int count,
                                          double* sh, double* tot) {
   constexpr int kG = BLOCK / kSlots;
   constexpr int kW = BLOCK / 64;
@@ -1252,7 +1258,7 @@ __device__ inline void sum_partials_from(double (&v)[kSumBatch],
     for (int u = 0; u < w; ++u) v[u] = v[u] + v[u + w];
   s += v[0];
   for (int b0 = grp + kSumBatch * kG; b0 < count; b0 += kSumBatch * kG) {
-    load_partial_batch<BLOCK>(partials, count, b0, v);
+    load_partial_batch<BLOCK, P>(partials, count, b0, v);
 #pragma unroll
     for (int w = kSumBatch / 2; w > 0; w /= 2)
 #pragma unroll
@@ -1276,12 +1282,12 @@ __device__ inline void sum_partials_from(double (&v)[kSumBatch],
   __syncthreads();
 }
 
-template <int BLOCK>
-__device__ inline void sum_partials(const double* partials, int count,
+template <int BLOCK, typename P>
+__device__ inline void sum_partials(const P* partials, int count,
                                     double* sh, double* tot) {
   double v[kSumBatch];
-  load_partial_batch<BLOCK>(partials, count, int(threadIdx.x) / kSlots, v);
-  sum_partials_from<BLOCK>(v, partials, count, sh, tot);
+  load_partial_batch<BLOCK, P>(partials, count, int(threadIdx.x) / kSlots, v);
+  sum_partials_from<BLOCK, P>(v, partials, count, sh, tot);
 }
 
 // ---------------------------------------------------------------------------
@@ -1546,8 +1552,8 @@ enum FinishMode { kStoreLinearization = 0, kUpdatePose = 1, kStoreInformation = 
 // (kUpdatePose), the stored linearisation, or the information matrix (the
 // partials of information_pixel). Called by every thread of one block of
 // BLOCK threads; `sh` is LDS of BLOCK doubles, `tot` of kSlots.
-template <int BLOCK>
-__device__ inline void finish_partials(stb_icp_state* st, const double* partials,
+template <int BLOCK, typename P>
+__device__ inline void finish_partials(stb_icp_state* st, const P* partials,
                                        int count, int mode, double gw,
                                        double fw, double* sh, double* tot) {
   const int t = threadIdx.x;
@@ -1562,7 +1568,7 @@ __device__ inline void finish_partials(stb_icp_state* st, const double* partials
     sel = st->map_sel;
     tag = uint32_t(st->map_epoch);
   }
-  sum_partials<BLOCK>(partials, count, sh, tot);
+  sum_partials<BLOCK, P>(partials, count, sh, tot);
   if (update) {
     if (t < 12) sh[t] = row;
     gn_finalize(st, tot, gw, fw, sh, iters, sel, tag);
