@@ -196,6 +196,37 @@ def test_fused_reduction_matches_oracle_sums(terms, size):
     assert rel(got_jr, Jr) < 1e-5, rel(got_jr, Jr)
 
 
+# The pipelined launch at other channel counts: one channel (only the last
+# channel's samples), four and five (the other channels in one batch of
+# four, with and without a padding slot).
+@pytest.mark.parametrize("channels", [1, 4, 5])
+def test_pipelined_launch_channel_counts_match_oracle_sums(channels):
+    P = prepared_pair(17, 640, 480, channels=channels)
+    T = small_motion(4, 1.0, 0.01)
+    gw, fw, dthr, athr, fthr = 1.0, 0.5, 0.1, ANG20, 2.75
+    g = to_dev(P, DEV)
+    level = Level(P["kcam"], g["src_pts"], g["src_nrm"], g["src_mask"],
+                  g["tgt_pts"], g["tgt_nrm"], g["tgt_mask"], g["src_feats"],
+                  g["tgt_feats"])
+    params = make_params(gw, fw, dthr, athr, fthr, True)
+    run = runner_for(DEV)
+    run.begin(level, torch.from_numpy(T))
+    run.linearize(level, params)
+    st = run.read_state()
+    cmap = cpu_map(P, T, dthr=dthr, athr=athr)
+    rt = T[:3].astype(np.float32)
+    jg, rg, _, cg = orc.geometric(P["src_pts"], P["src_mask"], rt, P["tgt_pts"],
+                                  P["tgt_nrm"], P["kcam"], cmap)
+    jf, rf, r2f, cf = orc.feature(P["src_pts"], P["src_feats"], P["src_mask"], rt,
+                                  P["tgt_feats"], P["kcam"], cmap, fthr)
+    JtJ = jg.sum(0, dtype=np.float64) * gw * gw + jf.sum(0, dtype=np.float64) * fw * fw
+    Jr = rg.sum(0, dtype=np.float64) * gw - rf.sum(0, dtype=np.float64) * fw
+    assert st.last_geom_count == cg and st.last_feat_count == cf
+    assert abs(st.last_feat_r2 - r2f.sum(dtype=np.float64)) <= 1e-5 * r2f.sum()
+    assert rel(np.array(st.last_jtj).reshape(6, 6), JtJ) < 1e-5
+    assert rel(np.array(st.last_jr), Jr) < 1e-5
+
+
 def test_fused_reduction_is_deterministic():
     P = prepared_pair(9, 320, 240)
     g = to_dev(P, DEV)
