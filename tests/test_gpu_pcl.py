@@ -231,3 +231,92 @@ def test_knn_grid_edge_cases():
         d1, i1 = knn(t(tg), t(q), 8, bound)
         assert torch.equal(i0, i1) and torch.equal(d0, d1), bound
         assert not bool(((i1 == 8) | (i1 == 9)).any())
+
+
+def _oracle_pcl_losses(tsrc, src_feats, tgt_pts, tgt_nrm, tgt_feats, bound,
+                       feat_thresh, alpha):
+    """FP64 restatement of the estimate_pcl cost (reference
+    autogradicp.py:183-201 on the correspondences of matching.py:127-172):
+    the nearest target within `bound` of each transformed source point (the
+    oracle's exact k-nearest search, k = 8), point-to-plane cost against its
+    point and normal, and the Huber cost of the distance between the source
+    features and the target features interpolated from the 8 neighbours."""
+    d, i = orc.knn(tgt_pts.astype(tsrc.dtype), tsrc, 8, bound)
+    m = tgt_pts.shape[0]
+    mask = i[:, 0] < m
+    top1 = i[mask, 0]
+    s = tsrc[mask].astype(np.float64)
+    cost = (tgt_nrm[top1].astype(np.float64)
+            * (tgt_pts[top1].astype(np.float64) - s)).sum(1)
+    geom = float(np.mean(cost ** 2))
+    f = orc.nn_forward(d.astype(np.float64), i, tgt_feats.astype(np.float64))
+    fd = np.linalg.norm(f[:, mask] - src_feats[:, mask].astype(np.float64), axis=0)
+    fd = fd[fd ** 2 < feat_thresh ** 2]
+    lo, hi = fd[fd < alpha], fd[fd >= alpha]
+    feat = (lo.mean() if lo.size else 0.0) + (
+        (alpha * np.abs(hi) - alpha).mean() if hi.size else 0.0)
+    return geom, feat, mask, top1
+
+
+def test_estimate_pcl_objective_and_gradient_match_the_oracle():
+    """The cost that AutogradICP.estimate_pcl hands its optimiser, at the
+    identity and at a perturbed pose: the transformed source points against
+    the oracle's FP64 rigid transform, the matches and both cost terms
+    against the FP64 restatement above (its neighbour search run on the same
+    FP32 points, so that the decisions are shared), and the autograd
+    gradient of the geometric term against FP64 central differences of the
+    restated cost with the matches held."""
+    from slamtb.camera import RigidTransform
+    from slamtb.registration.autogradicp import _compacted_loss, _HuberLoss
+    from slamtb.registration.se3 import ExpRtToMatrix
+    from slamtb.spatial.matching import PointCloudMatcher
+    src, tgt, _ = pair_clouds()
+    src, tgt = src.downsample(0.01), tgt.downsample(0.01)
+    icp = AutogradICP(10, geom_weight=1.0, feat_weight=0.5, distance_threshold=0.1,
+                      feat_residual_thresh=2.75, huber_loss_alpha=0.3)
+    sp = src.points.cpu().numpy()
+    tp, tn = tgt.points.cpu().numpy(), tgt.normals.cpu().numpy()
+    sf, tf = src.features.cpu().numpy(), tgt.features.cpu().numpy()
+    rng = np.random.default_rng(8)
+    for x in (np.zeros(6), np.concatenate([rng.normal(0, 0.01, 3),
+                                           rng.normal(0, 0.01, 3)])):
+        xt = torch.tensor(x, dtype=torch.float32, device=DEV, requires_grad=True)
+        rigid = RigidTransform(ExpRtToMatrix.apply(xt))
+        tsrc = (rigid @ src.points).view(-1, 3)
+        with torch.no_grad():
+            tnormals = rigid.transform_normals(src.normals)
+        matcher = PointCloudMatcher(tgt.points, tgt.normals, tgt.features,
+                                    num_neighbors=8, distance_upper_bound=0.1)
+        found = matcher.find_correspondences(tsrc, tnormals)
+        geom, feat = _compacted_loss(tsrc, *found, src.features, icp,
+                                     _HuberLoss(icp.huber_loss_alpha))
+        geom.backward()
+        grad = xt.grad.double().cpu().numpy()
+        geom, feat = geom.detach(), feat.detach()
+        ts32 = tsrc.detach().cpu().numpy()
+
+        M = orc.exp_rt_to_matrix(x.astype(np.float64))
+        ts64 = sp.astype(np.float64) @ M[:, :3].T + M[:, 3]
+        assert np.abs(ts32 - ts64).max() <= 1e-5 * max(1.0, np.abs(ts64).max())
+        og, of, mask, top1 = _oracle_pcl_losses(ts32, sf, tp, tn, tf, 0.1, 2.75, 0.3)
+        assert np.array_equal(found[3].cpu().numpy(), mask)
+        assert mask.sum() > 0.5 * len(mask)
+        gv, fv = geom.item(), feat.item()
+        print(f"pose {x}: geom {gv:.9g} vs {og:.9g}, feat {fv:.9g} vs {of:.9g}")
+        assert abs(gv - og) <= 1e-5 * og
+        # (the Huber part alpha |d| - alpha is negative for d < 1)
+        assert abs(fv - of) <= 1e-5 * abs(of)
+
+        def g64(xx):
+            Mx = orc.exp_rt_to_matrix(xx)
+            s = sp[mask].astype(np.float64) @ Mx[:, :3].T + Mx[:, 3]
+            c = (tn[top1].astype(np.float64) * (tp[top1].astype(np.float64) - s)).sum(1)
+            return float(np.mean(c ** 2))
+        fd = np.empty(6)
+        for k in range(6):
+            e = np.zeros(6)
+            e[k] = 1e-6
+            fd[k] = (g64(x + e) - g64(x - e)) / 2e-6
+        rel = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
+        print(f"  gradient {grad} vs {fd}: {rel:.2e}")
+        assert rel < 1e-3
