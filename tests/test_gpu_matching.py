@@ -263,3 +263,74 @@ def test_rigid_transform_autograd():
     ref = (Mc[:3, :3].matmul(pc.view(-1, 3, 1)) + Mc[:3, 3].view(3, 1)).squeeze()
     torch.testing.assert_close((RigidTransform(M.detach()) @ p.detach()).cpu(), ref,
                                rtol=1e-12, atol=1e-12)
+
+
+def test_autogradicp_frame_objective_and_gradient_match_the_oracle(pair):
+    """The cost AutogradICP.estimate hands its optimiser (reference
+    autogradicp.py:183-201 over the FPCL matcher of matching.py:29-125), at
+    the true motion and at a perturbed pose: the matcher's rows against the
+    oracle's correspondence map and FPCL forward on the same FP32 points,
+    both cost terms against an FP64 restatement on those rows, and the
+    geometric term's autograd gradient against FP64 central differences
+    with the matches held."""
+    from slamtb.registration.autogradicp import _row_loss
+    from slamtb.registration.se3 import ExpRtToMatrix
+    P = pair
+    m = P["src_mask"].reshape(-1)
+    sp = P["src_pts"].reshape(-1, 3)[m]
+    sn = P["src_nrm"].reshape(-1, 3)[m]
+    sf = P["src_feats"].reshape(P["src_feats"].shape[0], -1)[:, m]
+    tp, tn, tf, k = P["tgt_pts"], P["tgt_nrm"], P["tgt_feats"], P["kcam"]
+    athr = math.pi / 4
+    icp = AutogradICP(10, geom_weight=1.0, feat_weight=0.5, distance_threshold=0.1,
+                      normals_angle_thresh=athr, feat_residual_thresh=2.0,
+                      huber_loss_alpha=0.3)
+    matcher = FramePointCloudMatcher(t(tp), t(tn), t(P["tgt_mask"]), t(tf),
+                                     KCamera(torch.from_numpy(k)), 0.1, athr)
+    x_true = orc.matrix_to_exp_rt(P["T_gt"])[0]
+    rng = np.random.default_rng(6)
+    for x in (x_true, x_true + np.concatenate([rng.normal(0, 0.01, 3),
+                                               rng.normal(0, 0.01, 3)])):
+        xt = torch.tensor(x, dtype=torch.float32, device=DEV, requires_grad=True)
+        rigid = RigidTransform(ExpRtToMatrix.apply(xt))
+        tsrc = (rigid @ t(sp)).view(-1, 3)
+        with torch.no_grad():
+            tnrm = rigid.transform_normals(t(sn))
+        match = matcher.match_rows(tsrc, tnrm)
+        geom, feat = _row_loss(tsrc, match, t(sf), icp)
+        geom.backward()
+        grad = xt.grad.double().cpu().numpy()
+        gv, fv = geom.item(), feat.item()
+        ts32, tn32 = tsrc.detach().cpu().numpy(), tnrm.cpu().numpy()
+
+        cmap = orc.corresp_map(ts32, tn32, np.ones(1, np.uint8), np.eye(4), tp, tn,
+                               P["tgt_mask"], k, 0.1, athr)
+        rp, rn, rf, rmask = orc.fpcl_forward(tp, tn, tf, ts32, k, cmap)
+        assert np.array_equal(match.mask.cpu().numpy(), rmask)
+        assert rmask.sum() > 0.5 * len(rmask)
+        cost = (rn[rmask].astype(np.float64)
+                * (rp[rmask].astype(np.float64) - ts32[rmask])).sum(1)
+        og = float(np.mean(cost ** 2))
+        d = np.linalg.norm(rf[:, rmask].astype(np.float64) - sf[:, rmask], axis=0)
+        d = d[d ** 2 < 2.0 ** 2]
+        lo, hi = d[d < 0.3], d[d >= 0.3]
+        of = (lo.mean() if lo.size else 0.0) + ((0.3 * hi - 0.3).mean() if hi.size else 0.0)
+        print(f"geom {gv:.9g} vs {og:.9g}; feat {fv:.9g} vs {of:.9g}")
+        assert abs(gv - og) <= 1e-5 * og
+        assert abs(fv - of) <= 1e-5 * abs(of)
+
+        sel = np.nonzero(rmask)[0]
+        q, nq = rp[sel].astype(np.float64), rn[sel].astype(np.float64)
+
+        def g64(xx):
+            M = orc.exp_rt_to_matrix(xx)
+            s = sp[sel].astype(np.float64) @ M[:, :3].T + M[:, 3]
+            return float(np.mean(((q - s) * nq).sum(1) ** 2))
+        fd = np.empty(6)
+        for i in range(6):
+            e = np.zeros(6)
+            e[i] = 1e-6
+            fd[i] = (g64(x + e) - g64(x - e)) / 2e-6
+        rel = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
+        print(f"  gradient {grad} vs {fd}: {rel:.2e}")
+        assert rel < 1e-3
