@@ -407,11 +407,13 @@ __global__ void __launch_bounds__(kBlock)
 // ---------------------------------------------------------------------------
 
 // Jacobian launch shape. By default at most kDefaultUpdateBlocks partials,
-// so that every block of the next correspondence launch sums them (64 KB)
-// cheaply: levels above kWidePixels use blocks of kSolveBlock threads (at
-// 640x480, 256 blocks of 1024 threads, the same occupancy as 1024 blocks of
-// 256), smaller ones blocks of kBlock. An explicit grid_update keeps blocks
-// of kBlock with that cap.
+// so that every block of the next correspondence launch sums them (32 KB)
+// cheaply: levels above kWidePixels use runs of pixels per block of
+// kSolveBlock threads (the same occupancy as four times as many blocks of
+// 256) or, for FP32 levels whose runs fit three pixels per thread, the
+// pipelined launch with blocks of kPipeBlock (at 640x480: 256 blocks of
+// 512 threads); smaller levels use blocks of kBlock. An explicit
+// grid_update keeps blocks of kBlock with that cap.
 constexpr int kDefaultUpdateBlocks = 256;
 constexpr int64_t kWidePixels = 2 * 65536;
 constexpr int kMaxUpdateBlocks = 4096;
@@ -456,7 +458,8 @@ Workspace carve(void* base, int32_t H, int32_t W) {
 // blocks looping over pixels. The kernel is bound by vector-instruction
 // issue once all its loads are in flight, and at 640x480 looping blocks beat
 // one-pixel blocks (7.3 against 7.9 us for 1024 against 1200 blocks of
-// 256): each block's reduction is shared by more pixels.
+// 256, with the one-pixel-per-thread kernel): each block's reduction is
+// shared by more pixels.
 struct JacShape {
   int block;
   int grid;

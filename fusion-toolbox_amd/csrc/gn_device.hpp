@@ -1188,7 +1188,7 @@ __device__ inline void wave_reduce_slots(float (&v)[kSlots], int lane) {
 // is LDS of BLOCK / 64 * kSlots doubles (kShared for kBlock). The
 // single-pair launches store FP32 partials (P = float): every block of the
 // loop's next correspondence launch reads all of them, and half the bytes
-// save ~0.45 us of a 14 us iteration at 640x480, while the rounding to FP32
+// save ~0.6 us of a 14 us iteration at 640x480, while the rounding to FP32
 // (6e-8 relative per block) matches that of the FP32 wavefront sums the
 // partial is made of. The sums over partials stay FP64.
 template <int BLOCK = kBlock, typename P>
