@@ -358,7 +358,7 @@ def midstop_pair():
 
 
 @pytest.mark.parametrize("case", ["geom320", "both640", "midstop", "odd1000x700",
-                                  "bf16_640"])
+                                  "bf16_640", "zero640"])
 def test_loop_equals_split_iterations(case):
     """stb_icp_iterate applies each update inside the next iteration's
     correspondence launch (every block sums the partials and solves; the
@@ -378,6 +378,10 @@ def test_loop_equals_split_iterations(case):
     elif case == "bf16_640":
         P, T0 = prepared_pair(7, 640, 480, channels=3), small_motion(5)
         gw, fw, iters, storage = 10.0, 1.0, 4, torch.bfloat16
+    elif case == "zero640":  # no overlap: the pipelined launch in a stopped loop
+        P, T0 = prepared_pair(7, 640, 480, channels=3), np.eye(4)
+        T0[:3, 3] = [0.0, 0.0, 50.0]
+        gw, fw, iters = 10.0, 1.0, 6
     else:
         P, T0 = midstop_pair()
         dthr = 0.05
@@ -409,6 +413,8 @@ def test_loop_equals_split_iterations(case):
         T = np.array(loop["transform"]).reshape(4, 4)
         err = orc.pose_error(T, ref["transform"])
         assert err[0] < 1e-6 and err[1] < 1e-6, err
+    elif case == "zero640":
+        assert loop["stopped"] == 1 and loop["iterations"] < iters
     else:
         assert loop["iterations"] == iters and loop["stopped"] == 0
 
