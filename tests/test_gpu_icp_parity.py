@@ -154,7 +154,7 @@ def rel(a, b):
 # 320x240: 256-thread Jacobian blocks. Above 131,072 pixels the FP32 launch
 # pipelines up to three pixels per thread over 256 runs of pixels: 560x320
 # leaves threads without a second pixel, 640x480 gives some wavefronts a
-# third, 640x560 most of them.
+# third; 640x560's runs (1,400 pixels) take the one-run-per-block kernel.
 @pytest.mark.parametrize("size", [(320, 240), (560, 320), (640, 480), (640, 560)])
 @pytest.mark.parametrize("terms", ["geom", "feat", "both"])
 def test_fused_reduction_matches_oracle_sums(terms, size):
