@@ -1432,7 +1432,11 @@ __device__ inline void store_linearization(stb_icp_state* st,
 // row-major). Returns false on a non-positive pivot (Tn is then not set).
 // Straight-line FP64 code with no memory traffic, so it runs at the same
 // speed in one thread or, redundantly, in every lane of a wavefront, and
-// both give the same bits.
+// both give the same bits. The update sits on every loop iteration's
+// critical path and its instruction count sets its length (FP64 issues at
+// the non-packed FP32 rate), so its products and sums are fused
+// multiply-adds (no reference path matches this solve's bits; both entry
+// paths share it).
 __device__ inline bool gn_update_f64(const double* tot, const double* T,
                                      double Tn[12]) {
   double A[21], b[6];
@@ -1450,7 +1454,7 @@ __device__ inline bool gn_update_f64(const double* tot, const double* T,
   for (int j = 0; j < 6; ++j) {
     double d2 = A[tri_up(j, j)];
 #pragma unroll
-    for (int q = 0; q < j; ++q) d2 -= L[tri_lo(j, q)] * L[tri_lo(j, q)];
+    for (int q = 0; q < j; ++q) d2 = fma(-L[tri_lo(j, q)], L[tri_lo(j, q)], d2);
     ok = ok && (d2 > 0.0);
     double r = __builtin_amdgcn_rsq(d2);
     r = fma(r * 0.5, fma(-d2 * r, r, 1.0), r);
@@ -1460,7 +1464,7 @@ __device__ inline bool gn_update_f64(const double* tot, const double* T,
     for (int i = j + 1; i < 6; ++i) {
       double v = A[tri_up(j, i)];
 #pragma unroll
-      for (int q = 0; q < j; ++q) v -= L[tri_lo(i, q)] * L[tri_lo(j, q)];
+      for (int q = 0; q < j; ++q) v = fma(-L[tri_lo(i, q)], L[tri_lo(j, q)], v);
       L[tri_lo(i, j)] = v * rinv[j];
     }
   }
@@ -1470,14 +1474,14 @@ __device__ inline bool gn_update_f64(const double* tot, const double* T,
   for (int i = 0; i < 6; ++i) {  // L y = b
     double v = b[i];
 #pragma unroll
-    for (int q = 0; q < i; ++q) v -= L[tri_lo(i, q)] * y[q];
+    for (int q = 0; q < i; ++q) v = fma(-L[tri_lo(i, q)], y[q], v);
     y[i] = v * rinv[i];
   }
 #pragma unroll
   for (int i = 5; i >= 0; --i) {  // L^T x = y, the newest x[q] last
     double v = y[i];
 #pragma unroll
-    for (int q = 5; q > i; --q) v -= L[tri_lo(q, i)] * x[q];
+    for (int q = 5; q > i; --q) v = fma(-L[tri_lo(q, i)], x[q], v);
     x[i] = v * rinv[i];
   }
   double U[12];
@@ -1486,8 +1490,8 @@ __device__ inline bool gn_update_f64(const double* tot, const double* T,
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      double v = U[i * 4 + 0] * T[j] + U[i * 4 + 1] * T[4 + j] +
-                 U[i * 4 + 2] * T[8 + j];
+      double v = fma(U[i * 4 + 2], T[8 + j],
+                     fma(U[i * 4 + 1], T[4 + j], U[i * 4 + 0] * T[j]));
       if (j == 3) v += U[i * 4 + 3];
       Tn[i * 4 + j] = v;
     }
