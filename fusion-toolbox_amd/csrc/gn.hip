@@ -262,7 +262,8 @@ __global__ void __launch_bounds__(BLOCK)
 // with room in its registers for two pixels' gathered data. The block
 // partials keep their meaning (each block sums its run of pixels); only the
 // order of the FP32 lane sums differs from jacobian_fused_kernel's.
-// Requires chunk <= 3 * BLOCK (pipe_fits checks it on the host).
+// Requires chunk <= 3 * BLOCK (pipe_fits, on the host, takes runs of at
+// most 2.5 * BLOCK).
 template <bool GEOM, bool FEAT, int NB, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
     jacobian_pipe_kernel(stb_icp_level Ld, stb_icp_state* __restrict__ st,
@@ -410,7 +411,7 @@ __global__ void __launch_bounds__(kBlock)
 // so that every block of the next correspondence launch sums them (32 KB)
 // cheaply: levels above kWidePixels use runs of pixels per block of
 // kSolveBlock threads (the same occupancy as four times as many blocks of
-// 256) or, for FP32 levels whose runs fit three pixels per thread, the
+// 256) or, for FP32 levels whose runs fit 2.5 pixels per thread, the
 // pipelined launch with blocks of kPipeBlock (at 640x480: 256 blocks of
 // 512 threads); smaller levels use blocks of kBlock. An explicit
 // grid_update keeps blocks of kBlock with that cap.
