@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(kBlock, 4)
                           stb_icp_state* __restrict__ states,
                           unsigned long long* __restrict__ maps0,
                           unsigned long long* __restrict__ maps1,
-                          double* __restrict__ partials, TermWeights tw,
+                          float* __restrict__ partials, TermWeights tw,
                           int fixed_map) {
   __shared__ double sh[kShared];
   const PlanView pv = plan_view(plan);
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(kBlock, 4)
 __global__ void __launch_bounds__(kSolveBlock)
     batch_solve_kernel(const void* __restrict__ plan,
                        stb_icp_state* __restrict__ states,
-                       const double* __restrict__ partials, double gw,
+                       const float* __restrict__ partials, double gw,
                        double fw, int mode, int fixed_map) {
   __shared__ double sh[kSolveBlock];
   __shared__ double tot[kSlots];
@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(kBlock)
                              stb_icp_state* __restrict__ states,
                              unsigned long long* __restrict__ maps0,
                              unsigned long long* __restrict__ maps1,
-                             double* __restrict__ partials) {
+                             float* __restrict__ partials) {
   __shared__ double sh[kShared];
   const PlanView pv = plan_view(plan);
   for (int t = xcd_chunk(blockIdx.x, gridDim.x, kTileChunk); t < pv.tiles;
@@ -246,17 +246,17 @@ void launch_batch_jacobian(const void* plan, int num_pairs, int max_channels,
   if (FEAT && feat_layout == STB_FEAT_INTERLEAVED)
     hipLaunchKernelGGL((batch_jacobian_kernel<S, GEOM, FEAT, 4, true>),
                        dim3(grid), dim3(kBlock), 0, s, plan, states, sc.map0,
-                       sc.map1, sc.partials, tw, fixed);
+                       sc.map1, sc.partials32, tw, fixed);
   else if (FEAT && max_channels - 1 < 4)
     hipLaunchKernelGGL((batch_jacobian_kernel<S, GEOM, FEAT, 2>), dim3(grid),
                        dim3(kBlock), 0, s, plan, states, sc.map0, sc.map1,
-                       sc.partials, tw, fixed);
+                       sc.partials32, tw, fixed);
   else
     hipLaunchKernelGGL((batch_jacobian_kernel<S, GEOM, FEAT, 4>), dim3(grid),
                        dim3(kBlock), 0, s, plan, states, sc.map0, sc.map1,
-                       sc.partials, tw, fixed);
+                       sc.partials32, tw, fixed);
   hipLaunchKernelGGL(batch_solve_kernel, dim3(num_pairs), dim3(kSolveBlock), 0,
-                     s, plan, states, sc.partials, tw.gw, tw.fw, do_solve,
+                     s, plan, states, sc.partials32, tw.gw, tw.fw, do_solve,
                      fixed);
 }
 
@@ -290,9 +290,9 @@ void launch_batch(const void* plan, const stb_icp_batch_sizes& z,
                        plan, states, sc.map0, sc.map1, dthr2, cstar, fixed);
     if (phase == kInformation) {
       hipLaunchKernelGGL(batch_information_kernel<S>, dim3(gj), dim3(kBlock), 0,
-                         s, plan, states, sc.map0, sc.map1, sc.partials);
+                         s, plan, states, sc.map0, sc.map1, sc.partials32);
       hipLaunchKernelGGL(batch_solve_kernel, dim3(z.num_pairs),
-                         dim3(kSolveBlock), 0, s, plan, states, sc.partials,
+                         dim3(kSolveBlock), 0, s, plan, states, sc.partials32,
                          0.0, 0.0, int(kStoreInformation), 1);
       continue;
     }

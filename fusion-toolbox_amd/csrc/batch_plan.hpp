@@ -59,11 +59,15 @@ __device__ inline PlanView plan_view(const void* plan) {
   return v;
 }
 
-// Scratch: map0[total_map] | map1[total_map] | partials[tiles][32].
+// Scratch: map0[total_map] | map1[total_map] | partials[tiles][32]. The
+// forward launches keep FP32 tile partials (the per-pair solve reads all of
+// a pair's tiles every iteration), the adjoint's re-summed systems FP64: the
+// same region seen as either.
 struct Scratch {
   unsigned long long* map0;
   unsigned long long* map1;
   double* partials;
+  float* partials32;
   size_t bytes;
 };
 
@@ -76,6 +80,7 @@ inline Scratch carve_scratch(void* base, int P, int tiles, int64_t total_map) {
   s.map1 = reinterpret_cast<unsigned long long*>(p);
   p += align256(size_t(total_map) * 8);
   s.partials = reinterpret_cast<double*>(p);
+  s.partials32 = reinterpret_cast<float*>(p);
   p += align256(size_t(tiles) * kSlots * sizeof(double));
   s.bytes = size_t(p - static_cast<char*>(base));
   return s;
