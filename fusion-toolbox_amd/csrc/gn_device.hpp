@@ -6,9 +6,9 @@
 //   * the slot reduction: 21 unique JtJ + 6 Jr + r2/count bookkeeping per
 //     lane in FP32, recursive halving across the wavefront (permlane swaps
 //     and DPP), FP64 across waves through LDS, one FP32 partial per block
-//     (FP64 for the adjoint's systems), and a fixed-order FP64 sum of the partials
-//     in a following launch (by one block, or by every block of a loop's
-//     next correspondence launch);
+//     (FP64 for the adjoint's systems), and a fixed-order FP64 sum of the
+//     partials in a following launch (by one block, or by every block of a
+//     loop's next correspondence launch);
 //   * the per-pair state update: FP64 Cholesky solve and exp_rt_to_matrix,
 //     in one thread or redundantly in every lane of one wavefront.
 #pragma once
