@@ -152,10 +152,15 @@ def rel(a, b):
 
 
 # 320x240: 256-thread Jacobian blocks. Above 131,072 pixels the FP32 launch
-# pipelines up to three pixels per thread over 256 runs of pixels: 560x320
-# leaves threads without a second pixel, 640x480 gives some wavefronts a
-# third; 640x560's runs (1,400 pixels) take the one-run-per-block kernel.
-@pytest.mark.parametrize("size", [(320, 240), (560, 320), (640, 480), (640, 560)])
+# pipelines up to three pixels per thread over runs of pixels (512-thread
+# blocks): 500x300 (runs of 1,021, the last 934) leaves threads without a
+# second pixel, 560x320 (1,024) gives every thread exactly two, 640x480
+# (1,200) gives three wavefronts of a block a third, 640x512 (1,280, the
+# longest pipelined run) four, and 601x457 (1,073, the last 1,042; odd
+# width) part of one; 640x560's runs (1,400) take the one-run-per-block
+# kernel.
+@pytest.mark.parametrize("size", [(320, 240), (500, 300), (560, 320), (640, 480),
+                                  (640, 512), (601, 457), (640, 560)])
 @pytest.mark.parametrize("terms", ["geom", "feat", "both"])
 def test_fused_reduction_matches_oracle_sums(terms, size):
     P = prepared_pair(9, size[0], size[1], channels=3)
