@@ -135,6 +135,96 @@ __global__ void __launch_bounds__(kBlock, 4)
   }
 }
 
+// batch_jacobian_kernel as a software pipeline over a thread's four pixels
+// of a tile of kPipeTile pixels (FP32 storage; planar features whose
+// channels other than the last fit one batch of NB, or none): jac_issue /
+// jac_consume as in jacobian_pipe_kernel, each next pixel's gathers issued
+// before the current pixel's terms, so that they are in flight while the
+// VALU works instead of after it. The same per-pixel products and
+// decisions; only the order of the FP32 lane sums differs from
+// batch_jacobian_kernel's. Pixels past the pair's source count evaluate as
+// invalid (their buffer loads return 0). Two pixels' gathered data in
+// registers (132 VGPRs with three-channel colour) leave three wavefronts per
+// SIMD; 64 mixed pairs ran 9,031-9,067 pairs/s this way against 8,837-8,853
+// with batch_jacobian_kernel, 8,670-8,700 when bounded to four wavefronts
+// (four registers spilled), and 8,880-8,900 with the third and fourth
+// pixels' source loads issued later, behind the gathers.
+constexpr int kPipeTile = 4 * kBlock;
+
+template <bool GEOM, bool FEAT, int NB>
+__global__ void __launch_bounds__(kBlock, 3)
+    batch_jacobian_pipe_kernel(const void* __restrict__ plan,
+                               stb_icp_state* __restrict__ states,
+                               unsigned long long* __restrict__ maps0,
+                               unsigned long long* __restrict__ maps1,
+                               float* __restrict__ partials, TermWeights tw,
+                               int fixed_map) {
+  __shared__ double sh[kShared];
+  const PlanView pv = plan_view(plan);
+  for (int t = xcd_chunk(blockIdx.x, gridDim.x, kTileChunk); t < pv.tiles;
+       t += gridDim.x) {
+    const int p = pv.tile_pair[t];
+    const LevelView<float> L = view_of<float>(pv.levels[p]);
+    const LevelRes R = res_of(L);
+    const int first = pv.tile_start[p];
+    const int ntiles = pv.tile_start[p + 1] - first;
+    const int64_t base = int64_t(t - first) * kPipeTile;
+    // All four pixels' source data first, before the pair's state (written
+    // by the previous launch, an L2 miss) is read.
+    const int64_t i0 = base + threadIdx.x, i1 = i0 + kBlock, i2 = i1 + kBlock,
+                  i3 = i2 + kBlock;
+    const uint8_t vs0 = res_u8(R.sm, int(i0)), vs1 = res_u8(R.sm, int(i1));
+    const uint8_t vs2 = res_u8(R.sm, int(i2)), vs3 = res_u8(R.sm, int(i3));
+    const V3<float> ps0 = res_ld3<float>(R.sp, int(i0));
+    const V3<float> ps1 = res_ld3<float>(R.sp, int(i1));
+    const V3<float> ps2 = res_ld3<float>(R.sp, int(i2));
+    const V3<float> ps3 = res_ld3<float>(R.sp, int(i3));
+    __builtin_amdgcn_sched_barrier(0);
+    stb_icp_state* st = states + p;
+    if (!fixed_map && st->stopped) continue;
+    const int64_t off = pv.map_off[p];
+    const unsigned long long* map = pair_map(st, maps0, maps1, off, fixed_map);
+    const uint32_t tag = fixed_map ? kFreshTag : uint32_t(st->map_epoch);
+    if (!fixed_map && tag == 0) {
+      // As in batch_jacobian_kernel: this tile's share of the other map.
+      unsigned long long* next = pair_map(st, maps0, maps1, off, 1);
+      int64_t end = base + kPipeTile;
+      if (t - first == ntiles - 1 && end < L.HW) end = L.HW;
+      if (end > L.HW) end = L.HW;
+      for (int64_t i = base + threadIdx.x; i < end; i += blockDim.x)
+        next[i] = kKeySentinel;
+    }
+    float m[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
+    const __amdgpu_buffer_rsrc_t rmap = make_res(map, L.HW);
+    PxGather<NB> A, B;
+    float acc[kSlots];
+    jac_issue<GEOM, FEAT, NB>(L, R, m, rmap, i0, (i0 < L.n) & (vs0 != 0), ps0, A);
+    __builtin_amdgcn_sched_barrier(0);
+    jac_issue<GEOM, FEAT, NB>(L, R, m, rmap, i1, (i1 < L.n) & (vs1 != 0), ps1, B);
+    __builtin_amdgcn_sched_barrier(0);
+    jac_consume<GEOM, FEAT, true, NB>(L, A, uint32_t(i0), tag, tw, acc);
+    uint32_t k1 = uint32_t(i1);
+    settle(acc, B, k1);
+    __builtin_amdgcn_sched_barrier(0);
+    jac_issue<GEOM, FEAT, NB>(L, R, m, rmap, i2, (i2 < L.n) & (vs2 != 0), ps2, A);
+    __builtin_amdgcn_sched_barrier(0);
+    jac_consume<GEOM, FEAT, false, NB>(L, B, k1, tag, tw, acc);
+    uint32_t k2 = uint32_t(i2);
+    settle(acc, A, k2);
+    __builtin_amdgcn_sched_barrier(0);
+    jac_issue<GEOM, FEAT, NB>(L, R, m, rmap, i3, (i3 < L.n) & (vs3 != 0), ps3, B);
+    __builtin_amdgcn_sched_barrier(0);
+    jac_consume<GEOM, FEAT, false, NB>(L, A, k2, tag, tw, acc);
+    uint32_t k3 = uint32_t(i3);
+    settle(acc, B, k3);
+    __builtin_amdgcn_sched_barrier(0);
+    jac_consume<GEOM, FEAT, false, NB>(L, B, k3, tag, tw, acc);
+    block_partial(acc, partials + int64_t(t) * kSlots, sh, tw.b);
+  }
+}
+
 // One block per pair: the fixed-order sum of the pair's tile partials, then
 // its pose update, stored linearisation or information matrix (`mode`, a
 // FinishMode).
@@ -238,12 +328,25 @@ int grid_tiles(const stb_icp_params* P, int tiles, int dflt) {
 
 template <typename S, bool GEOM, bool FEAT>
 void launch_batch_jacobian(const void* plan, int num_pairs, int max_channels,
-                           int feat_layout, stb_icp_state* states,
+                           int feat_layout, int tile_px, stb_icp_state* states,
                            const Scratch& sc, int grid, const TermWeights& tw,
                            int do_solve, int fixed, hipStream_t s) {
-  // Batches of two feature channels when no pair has four besides its last;
+  // The pipelined kernel for FP32 tiles of kPipeTile pixels whose features
+  // (planar) have at most four channels besides the last; otherwise batches
+  // of two feature channels when no pair has four besides its last, and
   // interleaved features in chunks of eight.
-  if (FEAT && feat_layout == STB_FEAT_INTERLEAVED)
+  const bool pipe = std::is_same<S, float>::value && tile_px == kPipeTile &&
+                    (!FEAT || (feat_layout == STB_FEAT_PLANAR &&
+                               max_channels - 1 <= 4));
+  if (pipe && (!FEAT || max_channels - 1 <= 2))
+    hipLaunchKernelGGL((batch_jacobian_pipe_kernel<GEOM, FEAT, 2>), dim3(grid),
+                       dim3(kBlock), 0, s, plan, states, sc.map0, sc.map1,
+                       sc.partials32, tw, fixed);
+  else if (pipe)
+    hipLaunchKernelGGL((batch_jacobian_pipe_kernel<GEOM, FEAT, 4>), dim3(grid),
+                       dim3(kBlock), 0, s, plan, states, sc.map0, sc.map1,
+                       sc.partials32, tw, fixed);
+  else if (FEAT && feat_layout == STB_FEAT_INTERLEAVED)
     hipLaunchKernelGGL((batch_jacobian_kernel<S, GEOM, FEAT, 4, true>),
                        dim3(grid), dim3(kBlock), 0, s, plan, states, sc.map0,
                        sc.map1, sc.partials32, tw, fixed);
@@ -299,16 +402,16 @@ void launch_batch(const void* plan, const stb_icp_batch_sizes& z,
     const int solve = phase == kIterate ? 1 : 0;
     if (P.use_geom && P.use_feat)
       launch_batch_jacobian<S, true, true>(plan, z.num_pairs, z.max_channels,
-                                           z.feat_layout, states, sc, gj, tw,
-                                           solve, fixed, s);
+                                           z.feat_layout, z.tile_px, states, sc,
+                                           gj, tw, solve, fixed, s);
     else if (P.use_geom)
       launch_batch_jacobian<S, true, false>(plan, z.num_pairs, z.max_channels,
-                                            z.feat_layout, states, sc, gj, tw,
-                                            solve, fixed, s);
+                                            z.feat_layout, z.tile_px, states,
+                                            sc, gj, tw, solve, fixed, s);
     else
       launch_batch_jacobian<S, false, true>(plan, z.num_pairs, z.max_channels,
-                                            z.feat_layout, states, sc, gj, tw,
-                                            solve, fixed, s);
+                                            z.feat_layout, z.tile_px, states,
+                                            sc, gj, tw, solve, fixed, s);
   }
 }
 

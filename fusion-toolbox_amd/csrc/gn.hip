@@ -554,15 +554,16 @@ void launch_jacobian_nb(const stb_icp_level& L, const JacShape& js,
 // The pipelined Jacobian kernel's block, and whether it takes a launch:
 // FP32 storage, the wide-level shape (kSolveBlock blocks looping over runs of
 // pixels) with runs of at most 2.5 pixels per thread, and planar features
-// whose channels other than the last fit one batch. Longer runs give nearly
-// every wavefront a third pixel, whose terms have no loads left to overlap:
+// whose channels other than the last fit one batch of four. Longer runs give
+// nearly every wavefront a third pixel, whose terms have no loads left to
+// overlap:
 // at 720x540 (runs of 1519) the pipelined launch took 7.02 us against 6.61
 // for jacobian_fused_kernel, at 640x480 (runs of 1200) 5.93 against 6.06.
 constexpr int kPipeBlock = 512;
 
-bool pipe_fits(const stb_icp_level& L, const JacShape& js, bool feat, int nb) {
+bool pipe_fits(const stb_icp_level& L, const JacShape& js, bool feat) {
   if (L.storage != STB_F32 || js.block != kSolveBlock || js.one_px) return false;
-  if (feat && (L.feat_layout != STB_FEAT_PLANAR || L.channels - 1 > nb))
+  if (feat && (L.feat_layout != STB_FEAT_PLANAR || L.channels - 1 > 4))
     return false;
   const int64_t chunk = (L.n_src + js.grid - 1) / js.grid;
   return chunk <= 5 * kPipeBlock / 2;
@@ -591,10 +592,11 @@ void launch_jacobian(const stb_icp_level& L, const stb_icp_params& P,
   const JacShape js = jac_shape(P, L.n_src);
   // Feature channels other than the last load in batches of four, or of two
   // when there are fewer than four of them (three-channel colour: no
-  // padding slots); interleaved features in chunks of eight.
-  const int nb = L.channels - 1 < 4 ? 2 : 4;
-  if (std::is_same<S, float>::value && pipe_fits(L, js, FEAT, nb)) {
-    if (!FEAT || nb == 2)
+  // padding slots); interleaved features in chunks of eight. The pipelined
+  // launch loads them in one batch: of two when there are at most two, else
+  // of four.
+  if (std::is_same<S, float>::value && pipe_fits(L, js, FEAT)) {
+    if (!FEAT || L.channels - 1 <= 2)
       launch_jacobian_pipe<GEOM, FEAT, 2>(L, js, w, rec, tw, fixed_map, stream);
     else
       launch_jacobian_pipe<GEOM, FEAT, 4>(L, js, w, rec, tw, fixed_map, stream);

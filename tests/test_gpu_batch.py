@@ -99,7 +99,11 @@ def test_batch_graph_replay_is_repeatable():
             assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("storage,channels", [(torch.bfloat16, 3), (None, 16),
+# FP32 with four and five channels takes the pipelined Jacobian launch with
+# one batch of four (with and without a padding slot); sixteen channels and
+# 16-bit storage take batch_jacobian_kernel.
+@pytest.mark.parametrize("storage,channels", [(torch.bfloat16, 3), (None, 4),
+                                              (None, 5), (None, 16),
                                               (torch.float16, 16)])
 def test_batch_storage_and_channel_widths(storage, channels):
     """16-bit storage and wide descriptors (feature batches of four) through
@@ -110,6 +114,33 @@ def test_batch_storage_and_channel_widths(storage, channels):
     for i, levels in enumerate(pairs):
         ref = single(ms, levels)
         err = orc.pose_error(results[i].transform.numpy(), ref.transform.numpy())
+        assert err[0] < 1e-6 and err[1] < 1e-6, (i, err)
+        assert results[i].iterations == ref.iterations
+
+
+# Feature-only registration of the 80x60 pair of this seed goes astray (the
+# single-pair result ends 0.32 m from the truth with 39% of its pixels
+# matched); there a 1e-7 change in the order of the FP32 lane sums, which is
+# all that separates the pipelined batched launch from the single-pair one,
+# grows to 2e-4 rad, so the feature-only case batches the two pairs whose
+# registration holds.
+@pytest.mark.parametrize("gw,fw,sizes", [
+    (1.0, 0.0, [(160, 120), (320, 240), (80, 60)]),
+    (0.0, 1.0, [(160, 120), (320, 240)])])
+def test_batch_single_term_matches_single_pair_path(gw, fw, sizes):
+    """Geometric-only and feature-only registrations (the pipelined batched
+    Jacobian launch without one of its terms): equal to the single-pair
+    path on the same inputs."""
+    kw = dict(KW, geom_weight=gw, feat_weight=fw)
+    ms = MultiscaleRegistration([(1.0, ICPOdometry(6, **kw)),
+                                 (0.5, ICPOdometry(4, **kw)),
+                                 (0.5, ICPOdometry(3, **kw))])
+    ms, pairs, _ = make_pairs(sizes, seed0=90, ms=ms)
+    results = BatchedMultiscaleICP(ms.estimators, pairs, DEV).estimate()
+    for i, levels in enumerate(pairs):
+        ref = single(ms, levels)
+        err = orc.pose_error(results[i].transform.numpy(), ref.transform.numpy())
+        print(sizes[i], (gw, fw), "batched vs single", err, results[i].iterations)
         assert err[0] < 1e-6 and err[1] < 1e-6, (i, err)
         assert results[i].iterations == ref.iterations
 
