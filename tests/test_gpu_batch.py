@@ -79,6 +79,21 @@ def test_batch_matches_single_pair_path(tile_px):
         assert info_rel < 1e-4
 
 
+def test_batch_odd_sizes_match_single_pair_path():
+    """Odd widths and heights (bilinear taps at an odd right edge) and last
+    tiles of every length (161x121: 19,481 pixels, a last tile of 25; 331x247
+    and 97x73 at every level) through the pipelined batched launches."""
+    sizes = [(161, 121), (331, 247), (97, 73)]
+    ms, pairs, _ = make_pairs(sizes, seed0=110)
+    results = BatchedMultiscaleICP(ms.estimators, pairs, DEV).estimate()
+    for i, levels in enumerate(pairs):
+        ref = single(ms, levels)
+        err = orc.pose_error(results[i].transform.numpy(), ref.transform.numpy())
+        print(sizes[i], "batched vs single", err, results[i].iterations)
+        assert err[0] < 1e-6 and err[1] < 1e-6, (i, err)
+        assert results[i].iterations == ref.iterations
+
+
 def test_batch_graph_replay_is_repeatable():
     sizes = [(160, 120), (80, 60), (320, 240)]
     ms, pairs, _ = make_pairs(sizes, seed0=60)
