@@ -7,10 +7,9 @@
 // flattened tile list of all pairs (by default one workgroup per tile; with a
 // grid cap a workgroup walks tiles t = b, b + gridDim.x, ..., where b is its
 // XCD-chunked number xcd_chunk(blockIdx.x, gridDim.x, kTileChunk); every
-// Jacobian tile
-// leaves one FP64 partial), then one workgroup per pair sums that pair's
-// partials in tile order, solves its 6x6 system in FP64 and updates its pose.
-// All pairs
+// Jacobian tile leaves one partial, summed over the tile in FP64 and stored
+// in FP32), then one workgroup per pair sums that pair's partials in FP64 in
+// tile order, solves its 6x6 system in FP64 and updates its pose. All pairs
 // advance one iteration per three launches with no host involvement, and a
 // whole multi-level loop can be captured in one hipGraph.
 #include "batch_plan.hpp"

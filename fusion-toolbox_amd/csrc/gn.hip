@@ -1,9 +1,10 @@
 // Fused single-pair Gauss-Newton path (ICPOdometry): per iteration a
-// correspondence kernel and a Jacobian kernel that leaves one FP64 partial
-// per block. The partials are summed in a fixed order, the 6x6 system solved
-// and the pose updated on the device, by the next iteration's correspondence
-// launch (every block redundantly, see SolveRecord) or, after the last
-// iteration and in the split entry points, by a small launch of its own. (A
+// correspondence kernel and a Jacobian kernel that leaves one partial per
+// block (summed over the block in FP64, stored in FP32). The partials are
+// summed in a fixed order, the 6x6 system solved and the pose updated on the
+// device, by the next iteration's correspondence launch (every block
+// redundantly, see SolveRecord) or, after the last iteration and in the
+// split entry points, by a small launch of its own. (A
 // last-block ticket inside the Jacobian launch would cost every block a
 // device-scope fence and an atomic on one shared counter, about 20 ns per
 // block in all -- 10 us for 512 blocks.)
@@ -168,13 +169,14 @@ __global__ void __launch_bounds__(kSolveBlock)
     corresp_pixel(L, R, r.m, r.nm, i, map, use.tag, dthr2, cstar);
 }
 
-// Per-pixel Jacobians of one iteration; every block leaves one FP64 partial
-// for the launch that sums them, so no block waits on another. With ONE_PX
-// each thread owns at most one pixel (levels that fit the grid cap); larger
-// levels give block `blk` the blk-th run of `chunk` pixels (chunk computed
-// on the host). With `rec` (iterations after the first of a loop) the pose
-// and map come from the record of this iteration's correspondence launch,
-// and block 0 commits that record to the state. The source loads issue
+// Per-pixel Jacobians of one iteration; every block leaves one partial
+// (block_partial) for the launch that sums them, so no block waits on
+// another. With ONE_PX each thread owns at most one pixel (levels that fit
+// the grid cap); larger levels give block `blk` the blk-th run of `chunk`
+// pixels (chunk computed on the host). With `rec` (iterations after the
+// first of a loop) the pose and map come from the record of this
+// iteration's correspondence launch, and block 0 commits that record to the
+// state. The source loads issue
 // before any read of the state or the record: those were written by the
 // previous launch, miss this XCD's L2, and would otherwise hold the first
 // memory round trip back by one of their own.
@@ -354,7 +356,7 @@ __global__ void __launch_bounds__(kSolveBlock)
 }
 
 // G^T G of the matched pixels at the current pose over the spare map, one
-// FP64 partial per block; solve_fused_kernel (kStoreInformation) sums them.
+// partial per block; solve_fused_kernel (kStoreInformation) sums them.
 template <typename S>
 __global__ void __launch_bounds__(kBlock)
     information_fused_kernel(stb_icp_level Ld, stb_icp_state* __restrict__ st,

@@ -3,11 +3,12 @@ together on one GPU (the batch entry points of libslamtb_hip.so).
 
 For every pyramid level, all pairs' source pixels form one flattened tile
 list; each Gauss-Newton iteration is one correspondence launch and one
-Jacobian/reduction launch over that list, and the workgroup that completes
-a pair solves that pair's system and updates its pose on the device. The
-result equals running ``MultiscaleRegistration.estimate_frame`` on each
-pair with the same estimators (same per-pixel arithmetic; only the order of
-the FP64 partial sums differs).
+Jacobian launch over that list (one partial per tile), then one launch with
+a workgroup per pair that sums the pair's partials, solves its system and
+updates its pose on the device. The result equals running
+``MultiscaleRegistration.estimate_frame`` on each pair with the same
+estimators (same per-pixel arithmetic; only the order of the sums over
+pixels differs).
 """
 import ctypes
 

@@ -430,8 +430,9 @@ int stb_icp_begin(void* workspace, size_t workspace_bytes,
 
 /* Enqueues `num_iters` Gauss-Newton iterations in 2 * num_iters + 1
  * launches. Each iteration is a correspondence launch and a Jacobian launch
- * that leaves one FP64 partial per block; the partials are summed in a fixed
- * order, the 6x6 system solved by Cholesky in FP64 and exp_rt_to_matrix(x)
+ * that leaves one partial per block (summed over the block in FP64, stored
+ * in FP32); the partials are summed in FP64 in a fixed order, the 6x6
+ * system solved by Cholesky in FP64 and exp_rt_to_matrix(x)
  * left-multiplied into the pose by the next iteration's correspondence
  * launch (and, after the last iteration, by one solve launch). A failed
  * factorisation sets `stopped`, after which the remaining iterations change
