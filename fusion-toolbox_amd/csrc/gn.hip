@@ -437,7 +437,7 @@ struct Workspace {
 size_t ws_bytes(int32_t H, int32_t W) {
   const size_t map_len = size_t(H) * size_t(W);
   return align256(sizeof(stb_icp_state)) + 2 * align256(map_len * 8) +
-         align256(size_t(kMaxUpdateBlocks) * kSlots * sizeof(double)) +
+         align256(size_t(kMaxUpdateBlocks) * kSlots * sizeof(float)) +
          align256(sizeof(SolveRecord));
 }
 
@@ -452,7 +452,7 @@ Workspace carve(void* base, int32_t H, int32_t W) {
   w.map[1] = reinterpret_cast<unsigned long long*>(p);
   p += align256(size_t(w.map_len) * 8);
   w.partials = reinterpret_cast<float*>(p);
-  p += align256(size_t(kMaxUpdateBlocks) * kSlots * sizeof(double));
+  p += align256(size_t(kMaxUpdateBlocks) * kSlots * sizeof(float));
   w.rec = reinterpret_cast<SolveRecord*>(p);
   return w;
 }
