@@ -267,9 +267,11 @@ __global__ void __launch_bounds__(kBlock)
     float acc[kSlots];
 #pragma unroll
     for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
+    const LevelRes R = res_of(L);
+    const __amdgpu_buffer_rsrc_t rmap = make_res(map, L.HW);
     for (int k = threadIdx.x; k < pv.tile_px; k += blockDim.x) {
       const int64_t idx = base + k;
-      if (idx < L.n) information_pixel(L, m, map, kFreshTag, idx, acc);
+      if (idx < L.n) information_pixel(L, R, m, rmap, kFreshTag, idx, acc);
     }
     block_partial(acc, partials + int64_t(t) * kSlots, sh);
   }

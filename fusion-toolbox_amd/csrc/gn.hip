@@ -369,13 +369,15 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int i = 0; i < 12; ++i) m[i] = st->rt[i];
   const LevelView<S> L = view_of<S>(Ld);
+  const LevelRes R = res_of(L);
+  const __amdgpu_buffer_rsrc_t rmap = make_res(map, L.HW);
   float acc[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   for (int64_t idx = int64_t(blk) * blockDim.x + threadIdx.x; idx < L.n;
        idx += int64_t(gridDim.x) * blockDim.x)
-    information_pixel(L, m, map, kFreshTag, idx, acc);
+    information_pixel(L, R, m, rmap, kFreshTag, idx, acc);
   block_partial(acc, partials + int64_t(blk) * kSlots, sh);
 }
 

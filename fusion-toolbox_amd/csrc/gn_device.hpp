@@ -1082,21 +1082,34 @@ __device__ inline void jac_consume(const LevelView<float>& L,
 
 // G^T G of source pixel idx from its matched target point
 // (src/icp_jacobian_geometric.cu:117-158), parameter order [rot, trans].
+// Two memory round trips, as in the Jacobian pass: the source mask and point
+// load together, then the map key and the target point at the projected
+// pixel (target pixel 0 for a pixel that is not a valid source or projects
+// outside); a pixel that does not own its target pixel adds selected zeros.
+// The slots start at +0 and a sum of +0 and a finite term is that term, so
+// they come out as if those pixels were skipped.
 template <typename S>
-__device__ inline void information_pixel(const LevelView<S>& L, const float* m,
-                                         const unsigned long long* __restrict__ map,
+__device__ inline void information_pixel(const LevelView<S>& L,
+                                         const LevelRes& R, const float* m,
+                                         __amdgpu_buffer_rsrc_t rmap,
                                          uint32_t tag, int64_t idx,
                                          float (&acc)[kSlots]) {
 #pragma clang fp contract(off)
-  if (!L.sm[idx]) return;
-  const V3<float> p = rigid_point(m, ld3f(L.sp, idx));
+  const uint8_t vs = res_u8(R.sm, int(idx));
+  const V3<float> p = rigid_point(m, res_ld3<S>(R.sp, int(idx)));
   float u, v;
   project(L.fx, L.fy, L.cx, L.cy, p, u, v);
-  int ui, vi;
-  if (!round_in_range(u, L.W, ui) || !round_in_range(v, L.H, vi)) return;
-  const int64_t pos = int64_t(vi) * L.W + ui;
-  if (!key_owned(map[pos], uint32_t(idx), tag)) return;
-  const V3<float> q = ld3f(L.tp, pos);
+  int ui = 0, vi = 0;
+  const bool in_u = round_in_range(u, L.W, ui);
+  const bool in_v = round_in_range(v, L.H, vi);
+  const bool inside = (vs != 0) & in_u & in_v;
+  const int pos = inside ? mad24(vi, L.W, ui) : 0;
+  const uint32_t klo = __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos, 0, 0);
+  const uint32_t khi =
+      __builtin_amdgcn_raw_buffer_load_b32(rmap, 8 * pos + 4, 0, 0);
+  const V3<float> q = res_ld3<S>(R.tp, pos);
+  const bool own =
+      inside & key_owned((uint64_t(khi) << 32) | klo, uint32_t(idx), tag);
   const float r0[6] = {0.f, q.z, -q.y, 1.f, 0.f, 0.f};
   const float r1[6] = {-q.z, 0.f, q.x, 0.f, 1.f, 0.f};
   const float r2[6] = {q.y, -q.x, 0.f, 0.f, 0.f, 1.f};
@@ -1105,7 +1118,8 @@ __device__ inline void information_pixel(const LevelView<S>& L, const float* m,
   for (int a = 0; a < 6; ++a)
 #pragma unroll
     for (int b = a; b < 6; ++b) {
-      acc[k] += r0[a] * r0[b] + r1[a] * r1[b] + r2[a] * r2[b];
+      const float g = r0[a] * r0[b] + r1[a] * r1[b] + r2[a] * r2[b];
+      acc[k] += own ? g : 0.0f;
       ++k;
     }
 }
